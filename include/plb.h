@@ -1,0 +1,230 @@
+/*
+ * plb.h -- C ABI of libplb, the MI355X (gfx950) implementation of porelb's
+ * sparse-lattice D3Q19 time step.
+ *
+ * The reference (zhulianhua/porelb) has no library API: its solver is a set
+ * of `void f(SIMULATION*)` functions called from porelb.c main
+ * (porelb.h:127-165, porelb.c:402-443), its sparse index build is
+ * preprocess.c build_V/save_V (preprocess.c:29-193), and its halo exchange is
+ * MPI point-to-point (porelb.c:143-312).  Each entry point below names the
+ * reference code it replaces.  Plain C types only: pointers, sizes, codes.
+ *
+ * Ownership: the caller owns every host buffer; the library owns all device
+ * memory of a context.  Threading: one host thread per context; each context
+ * runs on its own HIP stream.  Errors: negative return codes (plb_strerror);
+ * the library never exits the process.
+ */
+#ifndef PLB_H
+#define PLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLB_Q 19
+#define PLB_ABI_VERSION 1
+
+/* ---- error codes (negative).  Where the reference exits with a code, the
+ * same value is used (porelb.c:18-22, 29-34, 120-137, 338-342, 406-410). */
+#define PLB_OK 0
+#define PLB_ERR_FILE (-1)      /* input file missing (porelb.c:18-22)          */
+#define PLB_ERR_NUM_P (-2)     /* rank-count / z-slab mismatch (porelb.c:29-34) */
+#define PLB_ERR_BOUNDS (-3)    /* node coordinate out of range (porelb.c:120-126) */
+#define PLB_ERR_PID (-4)       /* EXTERN neighbour pid invalid (porelb.c:128-137) */
+#define PLB_ERR_READ (-5)      /* short read (porelb.c:65-69)                  */
+#define PLB_ERR_DENSITY (-6)   /* density guard rho>1.2|rho<0.5 (porelb.c:338-342) */
+#define PLB_ERR_ARGS (-10)     /* bad command line (porelb.c:406-410)          */
+#define PLB_ERR_INVALID (-20)  /* invalid argument or configuration            */
+#define PLB_ERR_HIP (-21)      /* HIP runtime error                            */
+#define PLB_ERR_NOMEM (-22)    /* device or host allocation failed             */
+#define PLB_ERR_RANGE (-23)    /* node count beyond int32 node ids             */
+#define PLB_ERR_NODEVICE (-24) /* no usable GPU                                */
+#define PLB_ERR_FLAG (-25)     /* voxel flag outside {0,1}                     */
+
+/* ---- V.bin record (porelb.h:55-76; preprocess.h:10-22).  8-byte neighbour
+ * entries, 152-byte node records, little-endian; nb[j-1] describes the
+ * periodic neighbour at x+e_j. */
+enum { PLB_TYPE_FLUID = 0, PLB_TYPE_SOLID = 1, PLB_TYPE_EXTERN = 2 };
+typedef struct {
+    uint8_t node_type;
+    uint16_t node_pid;
+    uint32_t node_id;
+} plb_neighbor_info;
+typedef struct {
+    uint16_t x, y, z;
+    plb_neighbor_info nb_info[PLB_Q - 1];
+} plb_node_info;
+
+/* ---- configuration */
+enum { PLB_FP64 = 0, PLB_FP32 = 1, PLB_FP16 = 2 };
+/* Bounce-back: REFCOMPAT reproduces the reference's stream/bounce/receive
+ * overwrite order (porelb.c:285-312) exactly; HALFWAY is textbook half-way
+ * bounce-back (mass conserving). */
+enum { PLB_BB_REFCOMPAT = 0, PLB_BB_HALFWAY = 1 };
+/* Device layout of populations and pull table: SOA = [dir][node];
+ * AOSOA = tiles of 64 nodes, [tile][dir][64]. */
+enum { PLB_LAYOUT_SOA = 0, PLB_LAYOUT_AOSOA = 1 };
+/* Device node order: REFERENCE = V order (z,y,x lexicographic per rank);
+ * BRICK = bricks of brick[0] x brick[1] x brick[2] voxels in lexicographic
+ * brick order, z,y,x inside a brick.  Outputs are always in V order. */
+enum { PLB_ORDER_REFERENCE = 0, PLB_ORDER_BRICK = 1 };
+
+typedef struct {
+    double tau;          /* relaxation time (reference: 0.8, porelb.c:87)         */
+    double gx;           /* body force along x (reference: 1e-3, porelb.c:91)     */
+    double rho0;         /* initial density (reference: 1, porelb.c:228)          */
+    int precision;       /* PLB_FP64 | PLB_FP32 | PLB_FP16                        */
+    int bb_mode;         /* PLB_BB_REFCOMPAT | PLB_BB_HALFWAY                     */
+    int layout;          /* PLB_LAYOUT_SOA | PLB_LAYOUT_AOSOA                     */
+    int ordering;        /* PLB_ORDER_REFERENCE | PLB_ORDER_BRICK                 */
+    int brick[3];        /* brick edge lengths x,y,z for PLB_ORDER_BRICK          */
+    int block_size;      /* threads per block of the step kernel (64..1024)       */
+    int device;          /* HIP device ordinal                                    */
+    int fp16_scale_log2; /* FP16: stored = (f - w rho0) * 2^k; 0 = derive from gx */
+} plb_config;
+
+/* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
+ * REFERENCE order, 256 threads, device 0. */
+void plb_config_default(plb_config *cfg);
+
+typedef struct plb_ctx plb_ctx;
+
+/* ---- context creation.  Replaces alloc_memory + check_V +
+ * build_send_recv_info (porelb.c:15-81, 115-222): every rank's records of a
+ * V.bin are taken at once; links to other ranks (EXTERN) become global node
+ * ids in the pull table, so no halo pack/unpack exists on one GPU.  The
+ * checks of check_V are applied to all records (-3 / -4). */
+int plb_create(plb_ctx **out, const plb_config *cfg, const plb_node_info *V, int64_t n_nodes,
+               const int32_t *V_sub_start, int num_p, int nx, int ny, int nz);
+
+/* Same, straight from a voxel mask (u8 [nz][ny][nx], 1 = pore, 0 = solid,
+ * x fastest): the sparse index is built on the device (preprocess.c:29-164
+ * semantics with num_p z-slabs) without V records. */
+int plb_create_from_flag(plb_ctx **out, const plb_config *cfg, const uint8_t *flag, int nx,
+                         int ny, int nz, int num_p);
+
+void plb_destroy(plb_ctx *c);
+
+/* init_dist_macro (porelb.c:224-238): rho = rho0, u = 0, f = feq. */
+int plb_init_equilibrium(plb_ctx *c);
+
+/* nsteps time steps it0 .. it0+nsteps-1 of porelb.c main (porelb.c:426-437):
+ * stream + bounce-back + receive + collision fused in one kernel per step.
+ * `it` enters only through the reference's half-force shift for it > 0
+ * (porelb.c:347-348).  Macros are stored on the last step.  Blocks until the
+ * steps finish; returns PLB_ERR_DENSITY if the density guard tripped (the
+ * state is then frozen at the end of the faulting step). */
+int plb_step(plb_ctx *c, int64_t it0, int64_t nsteps);
+
+/* As plb_step, but enqueues on the context stream and returns at once;
+ * plb_sync waits and reports the density guard. */
+int plb_step_async(plb_ctx *c, int64_t it0, int64_t nsteps);
+int plb_sync(plb_ctx *c);
+
+/* Same as plb_step with HIP events recorded on the context stream around the
+ * launches; *ms = elapsed device time of the nsteps steps. */
+int plb_step_timed(plb_ctx *c, int64_t it0, int64_t nsteps, double *ms);
+
+/* Macros of the last stored step, V order (porelb.c:350-356; save_data
+ * writes them, porelb.c:375-391). Any pointer may be NULL. */
+int plb_read_macros(plb_ctx *c, double *rho, double *ux, double *uy, double *uz);
+
+/* Populations f0 after the last step, AoS [n][19] in V order, as FP64
+ * (shifted storage is un-shifted). */
+int plb_read_populations(plb_ctx *c, double *f_aos);
+
+/* Overwrites the populations (AoS [n][19], V order); used for
+ * checkpoint/resume (the reference's save_state/recovery_state stubs,
+ * porelb.c:367-373) and tests. */
+int plb_write_populations(plb_ctx *c, const double *f_aos);
+
+/* The pull table [18][n] in V order with V-order node ids; entry codes:
+ * >= 0 source node (pull direction j from it), -1 bounce (own node,
+ * direction re[j]), -2 constant tp[j]*rho0. */
+int plb_read_table(plb_ctx *c, int32_t *tab);
+
+int64_t plb_num_nodes(const plb_ctx *c);
+
+/* FP64 reductions (new: the reference has none; getK.c is an analytic
+ * calculator).  mass = sum of all populations; sum_ux = sum of the Ux macro;
+ * mean_ux = sum_ux / n; q = sum_ux / (nx ny nz); K = nu q / gx with
+ * nu = (tau - 1/2)/3 (porelb.c:89). */
+typedef struct {
+    double mass;
+    double sum_ux;
+    double mean_ux;
+    double q;
+    double K;
+    int64_t n_nodes;
+} plb_reduction;
+int plb_reduce(plb_ctx *c, plb_reduction *out);
+
+/* Density-guard record: first step with a violation, and at that step the
+ * lowest V-order node, its rank pid and local id, coordinates and rho
+ * (the reference's message, porelb.c:340). */
+typedef struct {
+    int tripped;
+    int64_t step;
+    int64_t node;
+    int pid;
+    int64_t local_id;
+    int x, y, z;
+    double rho;
+} plb_fault;
+int plb_last_fault(plb_ctx *c, plb_fault *out);
+
+/* Bytes per node update the step kernel moves as algorithmic traffic
+ * (populations read + written, pull-table entries read). */
+double plb_bytes_per_update(const plb_ctx *c);
+
+const char *plb_strerror(int code);
+
+/* ---- preprocess on the device (preprocess.c:29-164 build_V): flag ->
+ * V records (V order) and V_sub_start.  Call with V_out == NULL to get the
+ * pore count only.  Fields preprocess.c leaves uninitialised (SOLID pid/id,
+ * padding) are zero. */
+int plb_preprocess(const uint8_t *flag, int nx, int ny, int nz, int num_p, int device,
+                   plb_node_info *V_out, int32_t *V_sub_start_out, int64_t *n_nodes_out);
+
+/* ---- batched samples: many independent porous samples of uneven pore
+ * count advanced by one launch per step over all active samples, with
+ * per-sample convergence stopping. */
+typedef struct {
+    const uint8_t *flag; /* u8 [nz][ny][nx] */
+    int nx, ny, nz;
+    double tau, gx;
+} plb_sample;
+typedef struct plb_batch plb_batch;
+int plb_batch_create(plb_batch **out, const plb_config *cfg, const plb_sample *samples,
+                     int n_samples);
+/* Runs until every sample has converged (|d<u_x>|/|<u_x>| < tol between
+ * checks check_every steps apart) or max_steps.  steps_done[s] = steps the
+ * sample ran; updates[0] = total node updates performed. */
+int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
+                  int64_t *steps_done, int64_t *updates, double *ms);
+/* Per-sample reductions after a run (n_samples entries). */
+int plb_batch_reduce(plb_batch *b, plb_reduction *out);
+int64_t plb_batch_num_nodes(const plb_batch *b, int sample);
+void plb_batch_destroy(plb_batch *b);
+
+/* ---- synthetic voxel masks (inputs for tests and benchmarks; the reference
+ * reads a given flag.bin, preprocess.c:61-69).  Deterministic from seed.
+ * noise: i.i.d. N(0,1), periodic Gaussian blur sigma, the lowest
+ * round(porosity * V) values are pore (exact porosity).
+ * spheres: overlapping solid spheres, radius U[rmin,rmax], centres U[0,n)^3,
+ * periodic, added until porosity <= target.
+ * bcc: two solid spheres of radius sqrt(3) (n-1) chi / 4 per periodic cell of
+ * n^3 voxels at the corner and the centre (the getK.c geometry). */
+int plb_mask_noise(uint8_t *flag, int nx, int ny, int nz, double sigma, double porosity,
+                   uint64_t seed);
+int plb_mask_spheres(uint8_t *flag, int nx, int ny, int nz, double rmin, double rmax,
+                     double porosity, uint64_t seed);
+int plb_mask_bcc(uint8_t *flag, int n, double chi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

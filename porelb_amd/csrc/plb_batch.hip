@@ -1,0 +1,292 @@
+// plb_batch.hip -- many independent porous samples in one lattice.
+//
+// The samples' pore nodes are concatenated, each sample padded to a multiple
+// of the block size, so every block of the step kernel lies in one sample and
+// takes that sample's omega and body force.  One launch per time step covers
+// all active samples; the grid enumerates the blocks of active samples only
+// (a device list rebuilt when a sample stops), so a converged sample costs
+// nothing.  Every `check_every` steps the ux macros are stored and summed per
+// sample in FP64; a sample stops when |d<u_x>| / |<u_x>| < tol between two
+// checks.  A stopped sample's state stays in the ping-pong buffer that held
+// it at its last step (recorded per sample).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "plb_internal.hpp"
+
+using namespace plb;
+
+struct plb_batch {
+    plb_ctx* c = nullptr;
+    int S = 0;
+    std::vector<int64_t> off, n;
+    std::vector<int> nx, ny, nz;
+    std::vector<double> tau, gx;
+    std::vector<int32_t> block_sample;
+    std::vector<int> sample_cur;
+    std::vector<char> active;
+    int32_t* d_block_sample = nullptr;
+    int32_t* d_active = nullptr;
+    int64_t* d_end = nullptr;
+    int64_t* d_off = nullptr;
+    int64_t* d_n = nullptr;
+    double* d_omega = nullptr;
+    double* d_gx = nullptr;
+    double* d_sums = nullptr;
+};
+
+extern "C" void plb_batch_destroy(plb_batch* b)
+{
+    if (!b) return;
+    if (b->c) {
+        (void)hipSetDevice(b->c->device);
+        (void)hipStreamSynchronize(b->c->stream);
+    }
+    void* ptrs[] = {b->d_block_sample, b->d_active, b->d_end, b->d_off, b->d_n,
+                    b->d_omega, b->d_gx, b->d_sums};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    plb_destroy(b->c);
+    delete b;
+}
+
+static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* smp)
+{
+    plb_ctx* c = b->c;
+    const int bs = cfg->block_size;
+    std::vector<int32_t*> tabs(b->S, nullptr);
+    std::vector<uint64_t*> xyzs(b->S, nullptr);
+    int rc = PLB_OK;
+    int64_t total = 0;
+    for (int s = 0; s < b->S && !rc; s++) {
+        int64_t ns = 0;
+        std::vector<int32_t> start;
+        rc = build_ref_from_flag(smp[s].flag, smp[s].nx, smp[s].ny, smp[s].nz, 1, cfg->bb_mode,
+                                 c->stream, &tabs[s], &xyzs[s], &ns, &start);
+        b->off[s] = total;
+        b->n[s] = ns;
+        total += std::max<int64_t>(bs, (ns + bs - 1) / bs * bs);
+    }
+    if (!rc && total >= INT32_MAX) rc = PLB_ERR_RANGE;
+    if (!rc) {
+        c->N = total;
+        c->Npad = total;
+        hipError_t e = hipMalloc(&c->tab, sizeof(int32_t) * (Q - 1) * total);
+        if (e == hipSuccess) e = hipMemsetAsync(c->tab, 0, sizeof(int32_t) * (Q - 1) * total, c->stream);
+        if (e == hipSuccess) e = hipMalloc(&c->node_xyz, sizeof(uint64_t) * total);
+        if (e != hipSuccess) rc = e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
+    }
+    for (int s = 0; s < b->S && !rc; s++) {
+        rc = copy_table_offset(cfg->layout, tabs[s], b->n[s], b->off[s], c->tab, c->Npad, c->stream);
+        if (!rc && b->n[s] &&
+            hipMemcpyAsync(c->node_xyz + b->off[s], xyzs[s], sizeof(uint64_t) * b->n[s],
+                           hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            rc = PLB_ERR_HIP;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = PLB_ERR_HIP;
+    for (int s = 0; s < b->S; s++) {
+        if (tabs[s]) (void)hipFree(tabs[s]);
+        if (xyzs[s]) (void)hipFree(xyzs[s]);
+    }
+    return rc;
+}
+
+extern "C" int plb_batch_create(plb_batch** out, const plb_config* cfg, const plb_sample* samples,
+                                int n_samples)
+{
+    if (!out) return PLB_ERR_INVALID;
+    *out = nullptr;
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (n_samples < 1 || !samples) return PLB_ERR_INVALID;
+    if (cfg->ordering != PLB_ORDER_REFERENCE) return PLB_ERR_INVALID;
+    for (int s = 0; s < n_samples; s++) {
+        const plb_sample& p = samples[s];
+        if (!p.flag || p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || p.nx > 65535 || p.ny > 65535 ||
+            p.nz > 65535 || !(p.tau > 0.0) || !isfinite(p.gx))
+            return PLB_ERR_INVALID;
+    }
+    plb_batch* b = new plb_batch();
+    b->S = n_samples;
+    b->off.assign(n_samples, 0);
+    b->n.assign(n_samples, 0);
+    b->c = new plb_ctx();
+    rc = ctx_begin(b->c, cfg, samples[0].nx, samples[0].ny, samples[0].nz, 1);
+    if (!rc) rc = batch_build(b, cfg, samples);
+    if (!rc) rc = ctx_alloc_state(b->c);
+    if (!rc) {
+        const int bs = cfg->block_size;
+        const int64_t nblocks = b->c->Npad / bs;
+        b->block_sample.assign(nblocks, 0);
+        std::vector<int64_t> end(n_samples);
+        std::vector<double> om(n_samples), gx(n_samples);
+        for (int s = 0; s < n_samples; s++) {
+            const int64_t b0 = b->off[s] / bs;
+            const int64_t b1 = (s + 1 < n_samples ? b->off[s + 1] : b->c->Npad) / bs;
+            for (int64_t k = b0; k < b1; k++) b->block_sample[k] = s;
+            end[s] = b->off[s] + b->n[s];
+            om[s] = 1.0 / samples[s].tau;
+            gx[s] = samples[s].gx;
+            b->nx.push_back(samples[s].nx);
+            b->ny.push_back(samples[s].ny);
+            b->nz.push_back(samples[s].nz);
+            b->tau.push_back(samples[s].tau);
+            b->gx.push_back(samples[s].gx);
+        }
+        hipError_t e = hipMalloc(&b->d_block_sample, sizeof(int32_t) * nblocks);
+        if (e == hipSuccess) e = hipMalloc(&b->d_active, sizeof(int32_t) * nblocks);
+        if (e == hipSuccess) e = hipMalloc(&b->d_end, sizeof(int64_t) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_off, sizeof(int64_t) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_n, sizeof(int64_t) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_omega, sizeof(double) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_gx, sizeof(double) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_sums, sizeof(double) * n_samples);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_block_sample, b->block_sample.data(), sizeof(int32_t) * nblocks,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_end, end.data(), sizeof(int64_t) * n_samples, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_off, b->off.data(), sizeof(int64_t) * n_samples,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_n, b->n.data(), sizeof(int64_t) * n_samples, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_omega, om.data(), sizeof(double) * n_samples, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_gx, gx.data(), sizeof(double) * n_samples, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            plb_set_hip_error(e, "plb_batch_create", __FILE__, __LINE__);
+            rc = e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
+        }
+        b->sample_cur.assign(n_samples, 0);
+        b->active.assign(n_samples, 1);
+    }
+    if (rc) {
+        plb_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PLB_OK;
+}
+
+extern "C" int64_t plb_batch_num_nodes(const plb_batch* b, int s)
+{
+    if (!b || s < 0 || s >= b->S) return -1;
+    return b->n[s];
+}
+
+static int upload_active(plb_batch* b, int* n_active)
+{
+    std::vector<int32_t> list;
+    const int bs = b->c->cfg.block_size;
+    for (int s = 0; s < b->S; s++) {
+        if (!b->active[s]) continue;
+        const int64_t b0 = b->off[s] / bs;
+        const int64_t b1 = (b->off[s] + b->n[s] + bs - 1) / bs;
+        for (int64_t k = b0; k < b1; k++) list.push_back((int32_t)k);
+    }
+    *n_active = (int)list.size();
+    if (!list.empty())
+        PLB_HIP(hipMemcpyAsync(b->d_active, list.data(), sizeof(int32_t) * list.size(),
+                               hipMemcpyHostToDevice, b->c->stream));
+    PLB_HIP(hipStreamSynchronize(b->c->stream));
+    return PLB_OK;
+}
+
+extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_every, double tol,
+                             int64_t* steps_done, int64_t* updates, double* ms)
+{
+    if (!b || max_steps < 0 || check_every < 1 || !(tol >= 0.0)) return PLB_ERR_INVALID;
+    plb_ctx* c = b->c;
+    PLB_HIP(hipSetDevice(c->device));
+    int rc = plb_init_equilibrium(c);
+    if (rc) return rc;
+    b->active.assign(b->S, 1);
+    for (int s = 0; s < b->S; s++)
+        if (b->n[s] == 0) b->active[s] = 0;
+    std::vector<int64_t> done(b->S, 0);
+    std::vector<double> prev(b->S, NAN), sums(b->S, 0.0);
+    int n_active = 0;
+    rc = upload_active(b, &n_active);
+    if (rc) return rc;
+    int64_t upd = 0;
+    int64_t it = 0;
+    PLB_HIP(hipEventRecord(c->ev0, c->stream));
+    while (n_active > 0 && it < max_steps) {
+        const int64_t m = std::min(check_every, max_steps - it);
+        for (int64_t k = 0; k < m; k++) {
+            rc = launch_step_batch(c, it + k, k == m - 1, b->d_active, n_active, b->d_block_sample,
+                                   b->d_end, b->d_omega, b->d_gx);
+            if (rc) return rc;
+        }
+        it += m;
+        for (int s = 0; s < b->S; s++)
+            if (b->active[s]) {
+                done[s] += m;
+                upd += m * b->n[s];
+            }
+        rc = launch_sample_sums(c, b->d_off, b->d_n, b->S, b->d_sums);
+        if (rc) return rc;
+        unsigned long long key = NO_FAULT;
+        PLB_HIP(hipMemcpyAsync(sums.data(), b->d_sums, sizeof(double) * b->S, hipMemcpyDeviceToHost,
+                               c->stream));
+        PLB_HIP(hipMemcpyAsync(&key, c->fault, sizeof key, hipMemcpyDeviceToHost, c->stream));
+        PLB_HIP(hipStreamSynchronize(c->stream));
+        if (key != NO_FAULT) return PLB_ERR_DENSITY;
+        bool changed = false;
+        for (int s = 0; s < b->S; s++) {
+            if (!b->active[s]) continue;
+            const double mean = sums[s] / (double)b->n[s];
+            if (!isnan(prev[s]) && fabs(mean - prev[s]) <= tol * fabs(mean)) {
+                b->active[s] = 0;
+                b->sample_cur[s] = c->cur;
+                changed = true;
+            }
+            prev[s] = mean;
+        }
+        if (changed) {
+            rc = upload_active(b, &n_active);
+            if (rc) return rc;
+        }
+    }
+    for (int s = 0; s < b->S; s++)
+        if (b->active[s]) b->sample_cur[s] = c->cur;
+    PLB_HIP(hipEventRecord(c->ev1, c->stream));
+    PLB_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    PLB_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    if (ms) *ms = t;
+    if (steps_done)
+        for (int s = 0; s < b->S; s++) steps_done[s] = done[s];
+    if (updates) *updates = upd;
+    return PLB_OK;
+}
+
+extern "C" int plb_batch_reduce(plb_batch* b, plb_reduction* out)
+{
+    if (!b || !out) return PLB_ERR_INVALID;
+    plb_ctx* c = b->c;
+    PLB_HIP(hipSetDevice(c->device));
+    double wsum = 0.0;
+    for (int j = 0; j < Q; j++) wsum += TP[j] * c->cfg.rho0;
+    for (int s = 0; s < b->S; s++) {
+        double r[2] = {0.0, 0.0};
+        if (b->n[s] > 0) {
+            int rc = launch_reduce_range(c, b->sample_cur[s], b->off[s], b->n[s], true, r);
+            if (rc) return rc;
+        }
+        plb_reduction& o = out[s];
+        o.mass = r[0] + (c->cfg.precision != PLB_FP64 ? wsum * (double)b->n[s] : 0.0);
+        o.sum_ux = r[1];
+        o.n_nodes = b->n[s];
+        o.mean_ux = b->n[s] ? r[1] / (double)b->n[s] : 0.0;
+        o.q = r[1] / ((double)b->nx[s] * b->ny[s] * b->nz[s]);
+        const double nu = 1.0 / 3.0 * (b->tau[s] - 0.5);
+        o.K = b->gx[s] != 0.0 ? nu * o.q / b->gx[s] : 0.0;
+    }
+    return PLB_OK;
+}

@@ -1,0 +1,705 @@
+// plb_build.hip -- sparse index build on the device.
+//
+// Replaces, for the one-GPU case, three pieces of the reference:
+//  * preprocess.c build_V (preprocess.c:29-164): pore compaction in z,y,x
+//    order, z-slab rank = z/(nz/num_p), periodic neighbour classification;
+//  * the per-step decoding of 152-byte NODE_INFO records in
+//    stream_inner_nodes (porelb.c:285-300);
+//  * the halo bookkeeping of build_send_recv_info / update_recv_dist
+//    (porelb.c:143-222, 302-312): a link to another rank becomes a global
+//    node id in the pull table.
+//
+// Pull-table rule for slot (i, j), j >= 1, with dn = neighbour at x+e_j and
+// up = neighbour at x-e_j (V record entries nb[j-1] and nb[re[j]-1]):
+//   REFCOMPAT: up pore and (dn pore, or up on another rank, or up after i in
+//              its rank's local order)       -> stream from up
+//              else dn solid                 -> bounce (i, re[j])
+//              else                          -> constant tp[j]*rho0
+//   HALFWAY:   up pore -> stream from up, else bounce (i, re[j])
+// REFCOMPAT reproduces the last-writer-wins order of the reference's push
+// stream, its bounce rule f1[i][j] = f0[i][re[j]] (porelb.c:296-297), and
+// the receive that runs after the stream loop (porelb.c:428-433).
+//
+// Voxel masks are held as one bit per voxel (wave-wide __ballot over 64
+// voxels) plus an exclusive prefix of per-word pore counts, so the pore id
+// of any voxel is prefix[w] + popcount(bits[w] & below) -- 1.5 bits per
+// voxel instead of the 4-byte X_to_n array of preprocess.c:50.
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "plb_internal.hpp"
+
+namespace plb {
+
+// ---------------------------------------------------------------- scan
+constexpr int SCAN_T = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int64_t SCAN_PER_BLOCK = SCAN_T * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_T)
+k_scan_block(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t* bsum,
+             int64_t n)
+{
+    __shared__ uint64_t s[SCAN_T];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_PER_BLOCK + (int64_t)tid * SCAN_ITEMS;
+    uint64_t v[SCAN_ITEMS];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        sum += v[k];
+    }
+    s[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < SCAN_T; off <<= 1) {
+        uint64_t t = tid >= off ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += t;
+        __syncthreads();
+    }
+    uint64_t excl = s[tid] - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = excl;
+        excl += v[k];
+    }
+    if (bsum && tid == SCAN_T - 1) bsum[blockIdx.x] = s[tid];
+}
+
+__global__ void k_scan_add(uint64_t* out, const uint64_t* boff, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += boff[i / SCAN_PER_BLOCK];
+}
+
+int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return PLB_OK;
+    int64_t nb = (n + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(SCAN_T), 0, s, in, out, nullptr, n);
+        PLB_HIP(hipGetLastError());
+        return PLB_OK;
+    }
+    uint64_t *bsum = nullptr, *boff = nullptr;
+    PLB_HIP(hipMalloc(&bsum, sizeof(uint64_t) * nb));
+    PLB_HIP(hipMalloc(&boff, sizeof(uint64_t) * nb));
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, bsum, n);
+    PLB_HIP(hipGetLastError());
+    int rc = exclusive_scan_u64(bsum, boff, nb, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, boff,
+                       n);
+    PLB_HIP(hipGetLastError());
+    PLB_HIP(hipStreamSynchronize(s));
+    PLB_HIP(hipFree(bsum));
+    PLB_HIP(hipFree(boff));
+    return PLB_OK;
+}
+
+// ---------------------------------------------------------------- bitmask helpers
+__device__ __forceinline__ bool is_pore(const uint64_t* bits, int64_t v)
+{
+    return (bits[v >> 6] >> (v & 63)) & 1ull;
+}
+__device__ __forceinline__ int64_t pore_id(const uint64_t* bits, const uint64_t* wpre, int64_t v)
+{
+    const int64_t w = v >> 6;
+    const uint64_t below = (1ull << (v & 63)) - 1ull;
+    return (int64_t)wpre[w] + __popcll(bits[w] & below);
+}
+
+// flag (u8, x fastest) -> bits + per-word counts.  blockDim multiple of 64.
+__global__ void k_flag_bits(const uint8_t* __restrict__ flag, int64_t nvox, int64_t nwords,
+                            uint64_t* __restrict__ bits, uint64_t* __restrict__ wcount, int* err)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint8_t f = v < nvox ? flag[v] : 0;
+    if (f > 1) atomicCAS(err, 0, PLB_ERR_FLAG);
+    const uint64_t m = __ballot(f == 1);
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w = v >> 6;
+        if (w < nwords) {
+            bits[w] = m;
+            wcount[w] = __popcll(m);
+        }
+    }
+}
+
+__global__ void k_nodes_from_bits(const uint64_t* __restrict__ bits,
+                                  const uint64_t* __restrict__ wpre, int64_t nvox, int nx, int ny,
+                                  uint64_t* __restrict__ xyz)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox || !is_pore(bits, v)) return;
+    const int64_t id = pore_id(bits, wpre, v);
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t z = v / plane, r = v - z * plane;
+    const int64_t y = r / nx, x = r - y * nx;
+    xyz[id] = pack_xyz(x, y, z);
+}
+
+struct GridArgs {
+    const uint64_t* bits;
+    const uint64_t* wpre;
+    const uint64_t* xyz;
+    int64_t N;
+    int nx, ny, nz, slab;
+};
+
+__device__ __forceinline__ void unpack(uint64_t p, int& x, int& y, int& z)
+{
+    x = (int)(p & 0xffff);
+    y = (int)((p >> 16) & 0xffff);
+    z = (int)((p >> 32) & 0xffff);
+}
+__device__ __forceinline__ int64_t lin(const GridArgs& a, int x, int y, int z)
+{
+    return ((int64_t)z * a.ny + y) * a.nx + x;
+}
+__device__ __forceinline__ int wrap(int v, int n) { return (v + n) % n; }
+
+// Pull table straight from the voxel bitmask (V order, [18][N]).
+__global__ void k_table_from_bits(GridArgs a, int mode, int32_t* __restrict__ tab)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.N) return;
+    int x, y, z;
+    unpack(a.xyz[g], x, y, z);
+    const int p = z / a.slab;
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const int64_t vd = lin(a, wrap(x + EX[j], a.nx), wrap(y + EY[j], a.ny), wrap(z + EZ[j], a.nz));
+        const int zu = wrap(z - EZ[j], a.nz);
+        const int64_t vu = lin(a, wrap(x - EX[j], a.nx), wrap(y - EY[j], a.ny), zu);
+        const bool up = is_pore(a.bits, vu);
+        int32_t t;
+        if (mode == PLB_BB_HALFWAY) {
+            t = up ? (int32_t)pore_id(a.bits, a.wpre, vu) : TAB_BOUNCE;
+        } else {
+            const bool dn = is_pore(a.bits, vd);
+            const int64_t uid = up ? pore_id(a.bits, a.wpre, vu) : -1;
+            if (up && (dn || (zu / a.slab) != p || uid > g))
+                t = (int32_t)uid;
+            else if (!dn)
+                t = TAB_BOUNCE;
+            else
+                t = TAB_CONST;
+        }
+        tab[(int64_t)(j - 1) * a.N + g] = t;
+    }
+}
+
+// V records from the voxel bitmask (device preprocess; preprocess.c:124-157).
+__global__ void k_V_from_bits(GridArgs a, const int32_t* __restrict__ start,
+                              uint64_t* __restrict__ Vw)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.N) return;
+    int x, y, z;
+    unpack(a.xyz[g], x, y, z);
+    const int p = z / a.slab;
+    uint64_t* rec = Vw + g * 19;
+    rec[0] = pack_xyz(x, y, z);
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const int zd = wrap(z + EZ[j], a.nz);
+        const int64_t vd = lin(a, wrap(x + EX[j], a.nx), wrap(y + EY[j], a.ny), zd);
+        uint64_t w;
+        if (!is_pore(a.bits, vd)) {
+            w = (uint64_t)PLB_TYPE_SOLID;
+        } else {
+            const int pp = zd / a.slab;
+            const int64_t local = pore_id(a.bits, a.wpre, vd) - start[pp];
+            const uint64_t type = (pp == p) ? PLB_TYPE_FLUID : PLB_TYPE_EXTERN;
+            w = type | ((uint64_t)pp << 16) | ((uint64_t)local << 32);
+        }
+        rec[j] = w;
+    }
+}
+
+__global__ void k_slab_starts(GridArgs a, int num_p, int32_t* start)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= num_p) return;
+    const int64_t v = (int64_t)p * a.slab * a.nx * a.ny;
+    start[p] = (int32_t)pore_id(a.bits, a.wpre, v);
+}
+
+// ---------------------------------------------------------------- from V records
+struct VArgs {
+    const uint64_t* V;  // records as 19 words
+    int64_t N;
+    const int32_t* start;
+    int num_p;
+    int nx, ny, nz;
+    int mode;
+    int32_t* tab;
+    uint64_t* xyz;
+    int* err;
+};
+
+__device__ __forceinline__ int owner_rank(const int32_t* start, int num_p, int64_t g)
+{
+    int lo = 0, hi = num_p - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int64_t)start[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_table_from_V(VArgs a)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.N) return;
+    const uint64_t* rec = a.V + g * 19;
+    const uint64_t w0 = rec[0];
+    const int x = (int)(w0 & 0xffff), y = (int)((w0 >> 16) & 0xffff), z = (int)((w0 >> 32) & 0xffff);
+    // check_V (porelb.c:115-141), applied to every rank's records.
+    if (x >= a.nx || y >= a.ny || z >= a.nz) atomicCAS(a.err, 0, PLB_ERR_BOUNDS);
+    a.xyz[g] = pack_xyz(x, y, z);
+    const int p = owner_rank(a.start, a.num_p, g);
+    const int64_t k = g - a.start[p];
+    auto count = [&](int q) -> int64_t {
+        return (q + 1 < a.num_p ? (int64_t)a.start[q + 1] : a.N) - a.start[q];
+    };
+    uint64_t nb[Q];
+#pragma unroll
+    for (int j = 1; j < Q; j++) nb[j] = rec[j];
+    // Validate every entry once.
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const int type = (int)(nb[j] & 0xff);
+        const int pid = (int)((nb[j] >> 16) & 0xffff);
+        const int64_t id = (int64_t)(nb[j] >> 32);
+        if (type == PLB_TYPE_EXTERN) {
+            if (pid > a.num_p - 1 || pid == p) atomicCAS(a.err, 0, PLB_ERR_PID);
+            else if (id >= count(pid)) atomicCAS(a.err, 0, PLB_ERR_BOUNDS);
+        } else if (type == PLB_TYPE_FLUID) {
+            if (id >= count(p)) atomicCAS(a.err, 0, PLB_ERR_BOUNDS);
+        } else if (type != PLB_TYPE_SOLID) {
+            atomicCAS(a.err, 0, PLB_ERR_INVALID);
+        }
+    }
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const uint64_t dn = nb[j], up = nb[RE[j]];
+        const int dtype = (int)(dn & 0xff), utype = (int)(up & 0xff);
+        const int upid = (int)((up >> 16) & 0xffff);
+        const int64_t uid = (int64_t)(up >> 32);
+        int64_t ug = -1;
+        if (utype == PLB_TYPE_FLUID) ug = a.start[p] + uid;
+        if (utype == PLB_TYPE_EXTERN && upid < a.num_p) ug = a.start[upid] + uid;
+        if (ug >= a.N) ug = -1;  // invalid entries were flagged above
+        int32_t t;
+        if (a.mode == PLB_BB_HALFWAY) {
+            t = (utype != PLB_TYPE_SOLID && ug >= 0) ? (int32_t)ug : TAB_BOUNCE;
+        } else {
+            if (utype != PLB_TYPE_SOLID && ug >= 0 &&
+                (dtype != PLB_TYPE_SOLID || utype == PLB_TYPE_EXTERN || uid > k))
+                t = (int32_t)ug;
+            else if (dtype == PLB_TYPE_SOLID)
+                t = TAB_BOUNCE;
+            else
+                t = TAB_CONST;
+        }
+        a.tab[(int64_t)(j - 1) * a.N + g] = t;
+    }
+}
+
+// ---------------------------------------------------------------- ordering + layout
+struct BrickArgs {
+    int nx, ny, nz, bx, by, bz, nbx, nby, nbz;
+};
+__device__ __forceinline__ int64_t brick_lin(const BrickArgs& b, int x, int y, int z)
+{
+    const int64_t brick = ((int64_t)(z / b.bz) * b.nby + (y / b.by)) * b.nbx + (x / b.bx);
+    const int64_t inner = ((int64_t)(z % b.bz) * b.by + (y % b.by)) * b.bx + (x % b.bx);
+    return brick * ((int64_t)b.bx * b.by * b.bz) + inner;
+}
+
+__global__ void k_brick_bits(const uint64_t* __restrict__ xyz, int64_t N, BrickArgs b,
+                             unsigned long long* bits)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    int x, y, z;
+    unpack(xyz[g], x, y, z);
+    const int64_t v = brick_lin(b, x, y, z);
+    atomicOr(bits + (v >> 6), 1ull << (v & 63));
+}
+
+__global__ void k_popc_words(const uint64_t* bits, uint64_t* cnt, int64_t nw)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) cnt[w] = __popcll(bits[w]);
+}
+
+__global__ void k_brick_ids(const uint64_t* __restrict__ xyz_ref, int64_t N, BrickArgs b,
+                            const uint64_t* bits, const uint64_t* wpre, uint32_t* inv_perm,
+                            uint32_t* perm, uint64_t* xyz_int)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    int x, y, z;
+    const uint64_t p = xyz_ref[g];
+    unpack(p, x, y, z);
+    const int64_t k = pore_id(bits, wpre, brick_lin(b, x, y, z));
+    inv_perm[g] = (uint32_t)k;
+    perm[k] = (uint32_t)g;
+    xyz_int[k] = p;
+}
+
+template <int LAYOUT>
+__global__ void k_write_table(const int32_t* __restrict__ tabref, int64_t N,
+                              const uint32_t* __restrict__ inv_perm, int32_t* __restrict__ out,
+                              int64_t npad)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+#pragma unroll
+    for (int j = 0; j < Q - 1; j++) {
+        int32_t t = tabref[(int64_t)j * N + g];
+        if (t >= 0 && inv_perm) t = (int32_t)inv_perm[t];
+        out[tpos<LAYOUT>(j, k, npad)] = t;
+    }
+}
+
+static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref)
+{
+    const int64_t N = c->N;
+    hipStream_t s = c->stream;
+    const int64_t tab_elems = (Q - 1) * c->Npad;
+    PLB_HIP(hipMalloc(&c->tab, sizeof(int32_t) * tab_elems));
+    PLB_HIP(hipMemsetAsync(c->tab, 0, sizeof(int32_t) * tab_elems, s));
+    PLB_HIP(hipMalloc(&c->node_xyz, sizeof(uint64_t) * (N ? N : 1)));
+    if (c->cfg.ordering == PLB_ORDER_BRICK && N > 0) {
+        BrickArgs b;
+        b.nx = c->nx;
+        b.ny = c->ny;
+        b.nz = c->nz;
+        b.bx = c->cfg.brick[0];
+        b.by = c->cfg.brick[1];
+        b.bz = c->cfg.brick[2];
+        b.nbx = (c->nx + b.bx - 1) / b.bx;
+        b.nby = (c->ny + b.by - 1) / b.by;
+        b.nbz = (c->nz + b.bz - 1) / b.bz;
+        const int64_t nvb = (int64_t)b.nbx * b.nby * b.nbz * b.bx * b.by * b.bz;
+        const int64_t nw = (nvb + 63) / 64;
+        uint64_t *bits = nullptr, *cnt = nullptr, *wpre = nullptr;
+        PLB_HIP(hipMalloc(&bits, sizeof(uint64_t) * nw));
+        PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * nw));
+        PLB_HIP(hipMalloc(&wpre, sizeof(uint64_t) * nw));
+        PLB_HIP(hipMemsetAsync(bits, 0, sizeof(uint64_t) * nw, s));
+        hipLaunchKernelGGL(k_brick_bits, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b,
+                           (unsigned long long*)bits);
+        hipLaunchKernelGGL(k_popc_words, dim3(nblk(nw, 256)), dim3(256), 0, s, bits, cnt, nw);
+        PLB_HIP(hipGetLastError());
+        int rc = exclusive_scan_u64(cnt, wpre, nw, s);
+        if (rc) return rc;
+        PLB_HIP(hipMalloc(&c->perm, sizeof(uint32_t) * N));
+        PLB_HIP(hipMalloc(&c->inv_perm, sizeof(uint32_t) * N));
+        hipLaunchKernelGGL(k_brick_ids, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b, bits,
+                           wpre, c->inv_perm, c->perm, c->node_xyz);
+        PLB_HIP(hipGetLastError());
+        PLB_HIP(hipStreamSynchronize(s));
+        PLB_HIP(hipFree(bits));
+        PLB_HIP(hipFree(cnt));
+        PLB_HIP(hipFree(wpre));
+    } else if (N > 0) {
+        PLB_HIP(hipMemcpyAsync(c->node_xyz, xyz_ref, sizeof(uint64_t) * N,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    if (N > 0) {
+        if (c->cfg.layout == PLB_LAYOUT_SOA)
+            hipLaunchKernelGGL(k_write_table<PLB_LAYOUT_SOA>, dim3(nblk(N, 256)), dim3(256), 0, s,
+                               tabref, N, c->inv_perm, c->tab, c->Npad);
+        else
+            hipLaunchKernelGGL(k_write_table<PLB_LAYOUT_AOSOA>, dim3(nblk(N, 256)), dim3(256), 0,
+                               s, tabref, N, c->inv_perm, c->tab, c->Npad);
+        PLB_HIP(hipGetLastError());
+    }
+    PLB_HIP(hipStreamSynchronize(s));
+    return PLB_OK;
+}
+
+int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n)
+{
+    hipStream_t s = c->stream;
+    uint64_t* dV = nullptr;
+    int32_t *dstart = nullptr, *tabref = nullptr;
+    uint64_t* xyz = nullptr;
+    int* derr = nullptr;
+    const int64_t nn = n ? n : 1;
+    PLB_HIP(hipMalloc(&dV, sizeof(plb_node_info) * nn));
+    PLB_HIP(hipMalloc(&dstart, sizeof(int32_t) * c->num_p));
+    PLB_HIP(hipMalloc(&tabref, sizeof(int32_t) * (Q - 1) * nn));
+    PLB_HIP(hipMalloc(&xyz, sizeof(uint64_t) * nn));
+    PLB_HIP(hipMalloc(&derr, sizeof(int)));
+    PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+    if (n) PLB_HIP(hipMemcpyAsync(dV, V, sizeof(plb_node_info) * n, hipMemcpyHostToDevice, s));
+    PLB_HIP(hipMemcpyAsync(dstart, c->V_sub_start.data(), sizeof(int32_t) * c->num_p,
+                           hipMemcpyHostToDevice, s));
+    if (n) {
+        VArgs a;
+        a.V = dV;
+        a.N = n;
+        a.start = dstart;
+        a.num_p = c->num_p;
+        a.nx = c->nx;
+        a.ny = c->ny;
+        a.nz = c->nz;
+        a.mode = c->cfg.bb_mode;
+        a.tab = tabref;
+        a.xyz = xyz;
+        a.err = derr;
+        hipLaunchKernelGGL(k_table_from_V, dim3(nblk(n, 256)), dim3(256), 0, s, a);
+        PLB_HIP(hipGetLastError());
+    }
+    int herr = 0;
+    PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipStreamSynchronize(s));
+    PLB_HIP(hipFree(dV));
+    int rc = herr ? herr : finalize_table(c, tabref, xyz);
+    (void)hipFree(tabref);
+    (void)hipFree(xyz);
+    (void)hipFree(dstart);
+    (void)hipFree(derr);
+    return rc;
+}
+
+// Device-side mask analysis shared by plb_create_from_flag and plb_preprocess.
+struct FlagIndex {
+    uint64_t* bits = nullptr;
+    uint64_t* wpre = nullptr;
+    uint64_t* xyz = nullptr;
+    int64_t N = 0, nvox = 0, nwords = 0;
+    void release()
+    {
+        if (bits) (void)hipFree(bits);
+        if (wpre) (void)hipFree(wpre);
+        if (xyz) (void)hipFree(xyz);
+        bits = wpre = xyz = nullptr;
+    }
+};
+
+static int index_flag(const uint8_t* flag, int nx, int ny, int nz, hipStream_t s, FlagIndex& fi)
+{
+    fi.nvox = (int64_t)nx * ny * nz;
+    fi.nwords = (fi.nvox + 63) / 64;
+    uint8_t* dflag = nullptr;
+    uint64_t* cnt = nullptr;
+    int* derr = nullptr;
+    PLB_HIP(hipMalloc(&dflag, fi.nvox));
+    PLB_HIP(hipMalloc(&fi.bits, sizeof(uint64_t) * fi.nwords));
+    PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * fi.nwords));
+    PLB_HIP(hipMalloc(&fi.wpre, sizeof(uint64_t) * fi.nwords));
+    PLB_HIP(hipMalloc(&derr, sizeof(int)));
+    PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+    PLB_HIP(hipMemcpyAsync(dflag, flag, fi.nvox, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_flag_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, dflag, fi.nvox,
+                       fi.nwords, fi.bits, cnt, derr);
+    PLB_HIP(hipGetLastError());
+    int rc = exclusive_scan_u64(cnt, fi.wpre, fi.nwords, s);
+    if (rc) return rc;
+    uint64_t last_pre = 0, last_cnt = 0;
+    int herr = 0;
+    PLB_HIP(hipMemcpyAsync(&last_pre, fi.wpre + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&last_cnt, cnt + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipStreamSynchronize(s));
+    (void)hipFree(dflag);
+    (void)hipFree(cnt);
+    (void)hipFree(derr);
+    if (herr) return herr;
+    fi.N = (int64_t)(last_pre + last_cnt);
+    if (fi.N >= INT32_MAX) return PLB_ERR_RANGE;
+    PLB_HIP(hipMalloc(&fi.xyz, sizeof(uint64_t) * (fi.N ? fi.N : 1)));
+    hipLaunchKernelGGL(k_nodes_from_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, fi.bits,
+                       fi.wpre, fi.nvox, nx, ny, fi.xyz);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+// V-order pull table [18][N] and node coordinates straight from a voxel mask,
+// with num_p z-slabs (preprocess.c:97-117 rank rule).  Caller frees.
+int build_ref_from_flag(const uint8_t* flag, int nx, int ny, int nz, int num_p, int mode,
+                        hipStream_t s, int32_t** tabref_out, uint64_t** xyz_out, int64_t* n_out,
+                        std::vector<int32_t>* start_out)
+{
+    *tabref_out = nullptr;
+    *xyz_out = nullptr;
+    FlagIndex fi;
+    int rc = index_flag(flag, nx, ny, nz, s, fi);
+    if (rc) {
+        fi.release();
+        return rc;
+    }
+    GridArgs a;
+    a.bits = fi.bits;
+    a.wpre = fi.wpre;
+    a.xyz = fi.xyz;
+    a.N = fi.N;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.slab = nz / num_p;
+    int32_t* dstart = nullptr;
+    int32_t* tabref = nullptr;
+    hipError_t e = hipMalloc(&dstart, sizeof(int32_t) * num_p);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p, dstart);
+        start_out->assign(num_p, 0);
+        e = hipMemcpyAsync(start_out->data(), dstart, sizeof(int32_t) * num_p,
+                           hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipMalloc(&tabref, sizeof(int32_t) * (Q - 1) * (fi.N ? fi.N : 1));
+    if (e == hipSuccess && fi.N) {
+        hipLaunchKernelGGL(k_table_from_bits, dim3(nblk(fi.N, 256)), dim3(256), 0, s, a, mode,
+                           tabref);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (dstart) (void)hipFree(dstart);
+    (void)hipFree(fi.bits);
+    (void)hipFree(fi.wpre);
+    fi.bits = fi.wpre = nullptr;
+    if (e != hipSuccess) {
+        plb_set_hip_error(e, "build_ref_from_flag", __FILE__, __LINE__);
+        if (tabref) (void)hipFree(tabref);
+        fi.release();
+        return e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
+    }
+    *tabref_out = tabref;
+    *xyz_out = fi.xyz;
+    fi.xyz = nullptr;
+    *n_out = fi.N;
+    return PLB_OK;
+}
+
+int build_table_from_flag(plb_ctx* c, const uint8_t* flag)
+{
+    int32_t* tabref = nullptr;
+    uint64_t* xyz = nullptr;
+    int64_t n = 0;
+    int rc = build_ref_from_flag(flag, c->nx, c->ny, c->nz, c->num_p, c->cfg.bb_mode, c->stream,
+                                 &tabref, &xyz, &n, &c->V_sub_start);
+    if (rc) return rc;
+    c->N = n;
+    c->Npad = std::max<int64_t>(64, (n + 63) / 64 * 64);
+    rc = finalize_table(c, tabref, xyz);
+    (void)hipFree(tabref);
+    (void)hipFree(xyz);
+    return rc;
+}
+
+template <int LAYOUT>
+__global__ void k_copy_table_offset(const int32_t* __restrict__ tabref, int64_t n, int64_t off,
+                                    int32_t* __restrict__ out, int64_t npad)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+#pragma unroll
+    for (int j = 0; j < Q - 1; j++) {
+        int32_t t = tabref[(int64_t)j * n + g];
+        if (t >= 0) t += (int32_t)off;
+        out[tpos<LAYOUT>(j, off + g, npad)] = t;
+    }
+}
+
+// Places one sample's V-order table at node offset `off` of a batch table.
+int copy_table_offset(int layout, const int32_t* tabref, int64_t n, int64_t off, int32_t* out,
+                      int64_t npad, hipStream_t s)
+{
+    if (n == 0) return PLB_OK;
+    if (layout == PLB_LAYOUT_SOA)
+        hipLaunchKernelGGL(k_copy_table_offset<PLB_LAYOUT_SOA>, dim3(nblk(n, 256)), dim3(256), 0, s,
+                           tabref, n, off, out, npad);
+    else
+        hipLaunchKernelGGL(k_copy_table_offset<PLB_LAYOUT_AOSOA>, dim3(nblk(n, 256)), dim3(256), 0,
+                           s, tabref, n, off, out, npad);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+}  // namespace plb
+
+using namespace plb;
+
+extern "C" int plb_preprocess(const uint8_t* flag, int nx, int ny, int nz, int num_p, int device,
+                              plb_node_info* V_out, int32_t* V_sub_start_out,
+                              int64_t* n_nodes_out)
+{
+    if (!flag || nx <= 0 || ny <= 0 || nz <= 0 || nx > 65535 || ny > 65535 || nz > 65535)
+        return PLB_ERR_INVALID;
+    if (num_p < 1 || num_p > 65535 || nz / num_p == 0) return PLB_ERR_NUM_P;
+    const int slab = nz / num_p;
+    if ((nz - 1) / slab >= num_p) return PLB_ERR_NUM_P;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PLB_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return PLB_ERR_INVALID;
+    PLB_HIP(hipSetDevice(device));
+    hipStream_t s;
+    PLB_HIP(hipStreamCreate(&s));
+    FlagIndex fi;
+    int rc = index_flag(flag, nx, ny, nz, s, fi);
+    if (rc == PLB_OK) {
+        if (n_nodes_out) *n_nodes_out = fi.N;
+        GridArgs a;
+        a.bits = fi.bits;
+        a.wpre = fi.wpre;
+        a.xyz = fi.xyz;
+        a.N = fi.N;
+        a.nx = nx;
+        a.ny = ny;
+        a.nz = nz;
+        a.slab = slab;
+        int32_t* dstart = nullptr;
+        uint64_t* dV = nullptr;
+        hipError_t e = hipMalloc(&dstart, sizeof(int32_t) * num_p);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p,
+                               dstart);
+            if (V_sub_start_out)
+                e = hipMemcpyAsync(V_sub_start_out, dstart, sizeof(int32_t) * num_p,
+                                   hipMemcpyDeviceToHost, s);
+        }
+        if (e == hipSuccess && V_out && fi.N) {
+            // Records are produced in chunks so the device copy stays bounded.
+            const int64_t chunk = 16 * 1024 * 1024;
+            e = hipMalloc(&dV, sizeof(plb_node_info) * (fi.N < chunk ? fi.N : chunk));
+            for (int64_t b0 = 0; e == hipSuccess && b0 < fi.N; b0 += chunk) {
+                const int64_t m = (fi.N - b0 < chunk) ? fi.N - b0 : chunk;
+                GridArgs ac = a;
+                ac.xyz = fi.xyz + b0;
+                ac.N = m;
+                hipLaunchKernelGGL(k_V_from_bits, dim3(nblk(m, 256)), dim3(256), 0, s, ac, dstart,
+                                   dV);
+                e = hipGetLastError();
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(V_out + b0, dV, sizeof(plb_node_info) * m,
+                                       hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (dV) (void)hipFree(dV);
+        if (dstart) (void)hipFree(dstart);
+        if (e != hipSuccess) {
+            plb_set_hip_error(e, "plb_preprocess", __FILE__, __LINE__);
+            rc = (e == hipErrorOutOfMemory) ? PLB_ERR_NOMEM : PLB_ERR_HIP;
+        }
+    }
+    fi.release();
+    (void)hipStreamDestroy(s);
+    return rc;
+}

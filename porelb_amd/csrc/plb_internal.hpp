@@ -1,0 +1,134 @@
+// plb_internal.hpp -- shared definitions of libplb (gfx950).
+//
+// Device data layout (HBM):
+//   populations  f[2]   two ping-pong buffers of Npad*19 storage elements
+//                       (double, or float/half holding f - w_j*rho0)
+//   pull table   tab    Npad*18 int32: >=0 source node, -1 bounce, -2 const
+//   macros              [4][Npad] double (rho, ux, uy, uz), written on
+//                       request (last step of a call, convergence checks)
+//   perm/inv_perm       internal id <-> V-order id, only for BRICK order
+//   node_xyz            packed coordinates per internal id (x|y<<16|z<<32)
+// Layouts: SOA = [dir][node], AOSOA = [node/64][dir][64].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "plb.h"
+
+namespace plb {
+
+constexpr int Q = 19;
+
+// D3Q19 in the reference's direction order (porelb.h:10-50).
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, -1, 1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 0, 0, 0, 0};
+constexpr int RE[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr double TP[Q] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                          1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                          1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+
+constexpr int32_t TAB_BOUNCE = -1;
+constexpr int32_t TAB_CONST = -2;
+constexpr unsigned long long NO_FAULT = ~0ull;
+
+// Position of population (dir j, node n).
+template <int LAYOUT>
+__host__ __device__ __forceinline__ int64_t fpos(int j, int64_t n, int64_t npad)
+{
+    if (LAYOUT == PLB_LAYOUT_SOA) return (int64_t)j * npad + n;
+    return (n >> 6) * (Q * 64) + (int64_t)j * 64 + (n & 63);
+}
+// Position of pull-table entry (dir j = 1..18 stored at slot j-1, node n).
+template <int LAYOUT>
+__host__ __device__ __forceinline__ int64_t tpos(int jm1, int64_t n, int64_t npad)
+{
+    if (LAYOUT == PLB_LAYOUT_SOA) return (int64_t)jm1 * npad + n;
+    return (n >> 6) * ((Q - 1) * 64) + (int64_t)jm1 * 64 + (n & 63);
+}
+
+// Work-groups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Remap so that every XCD walks one contiguous stretch of
+// the node range: neighbouring tiles (which share cache lines of the gathered
+// populations) then meet in the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
+{
+    const int64_t per = nb >> 3, rem = nb & 7;
+    const int64_t x = b & 7, slot = b >> 3;
+    return x * per + (x < rem ? x : rem) + slot;
+}
+
+__host__ __device__ __forceinline__ uint64_t pack_xyz(uint64_t x, uint64_t y, uint64_t z)
+{
+    return x | (y << 16) | (z << 32);
+}
+
+}  // namespace plb
+
+struct plb_ctx {
+    plb_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t N = 0, Npad = 0;
+    int nx = 0, ny = 0, nz = 0, num_p = 1;
+    std::vector<int32_t> V_sub_start;
+    size_t elem = 8;            // bytes per stored population
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;                // f[cur] holds the current post-collision state
+    int32_t* tab = nullptr;     // internal layout
+    uint32_t* perm = nullptr;   // internal -> V order (null: identity)
+    uint32_t* inv_perm = nullptr;
+    uint64_t* node_xyz = nullptr;  // internal order
+    double* macros = nullptr;   // [4][Npad]
+    unsigned long long* fault = nullptr;
+    double* red = nullptr;      // reduction partials
+    int red_blocks = 0;
+    double omega = 0.0;
+    float scale = 1.f, inv_scale = 1.f;  // FP16 storage scale
+    int64_t last_it = -1;       // last step issued
+    int64_t pending_it0 = 0, pending_n = 0;
+    int cur_before = 0;
+};
+
+// Error helpers (host).
+#define PLB_HIP(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) {                                         \
+            plb_set_hip_error(e_, #expr, __FILE__, __LINE__);           \
+            return e_ == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP; \
+        }                                                               \
+    } while (0)
+
+void plb_set_hip_error(hipError_t e, const char* what, const char* file, int line);
+
+// Internal entry points shared between translation units.
+namespace plb {
+int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n);
+int build_table_from_flag(plb_ctx* c, const uint8_t* flag);
+int launch_step(plb_ctx* c, int64_t it, bool write_macros);
+int launch_reduce(plb_ctx* c, double* host_out2);
+int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s);
+int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref);
+int build_ref_from_flag(const uint8_t* flag, int nx, int ny, int nz, int num_p, int mode,
+                        hipStream_t s, int32_t** tabref_out, uint64_t** xyz_out, int64_t* n_out,
+                        std::vector<int32_t>* start_out);
+int copy_table_offset(int layout, const int32_t* tabref, int64_t n, int64_t off, int32_t* out,
+                      int64_t npad, hipStream_t s);
+int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* active,
+                      int n_active, const int32_t* block_sample, const int64_t* sample_end,
+                      const double* omega, const double* gx);
+int launch_init_eq(plb_ctx* c);
+int launch_reduce_range(plb_ctx* c, int buf, int64_t first, int64_t n, bool with_mass,
+                        double* out2_host);
+int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,
+                       double* d_out);
+int check_cfg(const plb_config* cfg);
+int ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, int num_p);
+int ctx_alloc_state(plb_ctx* c);
+}  // namespace plb

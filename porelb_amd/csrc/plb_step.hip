@@ -1,0 +1,569 @@
+// plb_step.hip -- the fused D3Q19 time step on gfx950, plus the kernels that
+// move populations and macros in and out of the device layouts and the FP64
+// reductions.
+//
+// One kernel per time step, one thread per pore node:
+//   1. 18 pull-table entries (int32, streamed with non-temporal loads);
+//   2. 19 population gathers issued back to back (stream: f[j][src];
+//      bounce: f[re[j]][self]; const: no data needed);
+//   3. collision with body force (porelb.c:318-365);
+//   4. 19 coalesced non-temporal stores into the other ping-pong buffer.
+// This fuses stream_inner_nodes, update_recv_dist and collision
+// (porelb.c:285-365); the reference's f1 array never exists.
+//
+// FP64 (PLB_FP64): unshifted populations, arithmetic in exactly the
+// reference's evaluation order (porelb.c:6-13, 326-362); the library is
+// compiled with -ffp-contract=off so results match the reference bit for bit.
+// FP32/FP16 storage: populations are stored as deviations d_j = f_j - w_j*rho0
+// (FP16 scaled by 2^k), and the collision runs in FP32 on the deviations:
+//   d'_j = (1-omega) d_j + omega*dfeq_j + S_j*feq_j,
+//   dfeq_j = w_j (drho + rho*P_j), P_j = 3 eu + 4.5 eu^2 - 1.5 u^2,
+//   S_j = (1 - omega/2)*3*(e_jx - u)*F,
+// which is the reference update minus w_j*rho0 on both sides.
+#include <algorithm>
+
+#include "plb_internal.hpp"
+
+namespace plb {
+
+struct StepArgs {
+    const void* fin;
+    void* fout;
+    const int32_t* tab;
+    int64_t N, Npad;
+    double omega, gx, rho0;
+    int64_t it;
+    int write_macros;
+    double* macros;  // [4][Npad]
+    unsigned long long* fault;
+    const uint32_t* perm;
+    float scale, inv_scale;
+};
+
+// ---------------------------------------------------------------- FP64, reference order
+__device__ __forceinline__ double feq_ref(int k, double rho, double u, double v, double w)
+{
+    // porelb.c:8-12, same operations in the same order.
+    double uv, eu, x;
+    eu = EX[k] * u + EY[k] * v + EZ[k] * w;
+    uv = u * u + v * v + w * w;
+    x = TP[k] * rho * (1.0 + 3 * eu + 4.5 * eu * eu - 1.5 * uv);
+    return x;
+}
+
+__device__ __forceinline__ void record_fault(const StepArgs& a, int64_t i, double rho)
+{
+    const uint64_t g = a.perm ? (uint64_t)a.perm[i] : (uint64_t)i;
+    const uint64_t key = ((uint64_t)a.it << 32) | g;
+    atomicMin(a.fault, (unsigned long long)key);
+    a.macros[i] = rho;  // rho of the offending node, read back by plb_last_fault
+}
+
+__device__ __forceinline__ bool skip_after_fault(const StepArgs& a)
+{
+    // A step after the first faulting step does nothing, so the state stays
+    // that of the faulting step (the reference exits there).
+    const unsigned long long fk = *a.fault;
+    return fk != NO_FAULT && (int64_t)(fk >> 32) < a.it;
+}
+
+// One node of the FP64 step, reference arithmetic.  The caller has checked
+// i < N.  a.omega / a.gx are the node's sample parameters.
+template <int LAYOUT>
+__device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i)
+{
+    const double* __restrict__ fin = (const double*)a.fin;
+    double* __restrict__ fout = (double*)a.fout;
+
+    int32_t t[Q];
+#pragma unroll
+    for (int j = 1; j < Q; j++) t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+    double f[Q];
+    f[0] = fin[fpos<LAYOUT>(0, i, a.Npad)];
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const int64_t src = t[j] >= 0 ? fpos<LAYOUT>(j, t[j], a.Npad) : fpos<LAYOUT>(RE[j], i, a.Npad);
+        f[j] = fin[src];
+    }
+#pragma unroll
+    for (int j = 1; j < Q; j++)
+        if (t[j] == TAB_CONST) f[j] = TP[j] * a.rho0 * 1.0;  // feq(rho0, u=0) = initial f1
+
+    // collision (porelb.c:326-362)
+    double rho = 0.0, rhou = 0.0, rhov = 0.0, rhow = 0.0;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        rho += f[j];
+        rhou += EX[j] * f[j];
+        rhov += EY[j] * f[j];
+        rhow += EZ[j] * f[j];
+    }
+    if ((rho > 1.2) | (rho < 0.5)) record_fault(a, i, rho);
+    if (a.it > 0) rhou += 0.5 * a.gx;
+    const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+    if (a.write_macros) {
+        a.macros[i] = rho;
+        a.macros[a.Npad + i] = u;
+        a.macros[2 * a.Npad + i] = v;
+        a.macros[3 * a.Npad + i] = w;
+    }
+    const double omega = a.omega, Xforce = a.gx;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const double o = (1.0 - omega) * f[j] +
+                         (omega + (1.0 - 0.5 * omega) * 3.0 * (EX[j] - u) * Xforce) *
+                             feq_ref(j, rho, u, v, w);
+        __builtin_nontemporal_store(o, fout + fpos<LAYOUT>(j, i, a.Npad));
+    }
+}
+
+// ---------------------------------------------------------------- shifted FP32 / FP16 storage
+__device__ __forceinline__ float ld_dev(const float* p, int64_t k, float) { return p[k]; }
+__device__ __forceinline__ float ld_dev(const __half* p, int64_t k, float inv)
+{
+    return __half2float(p[k]) * inv;
+}
+__device__ __forceinline__ void st_dev(float* p, int64_t k, float v, float)
+{
+    __builtin_nontemporal_store(v, p + k);
+}
+__device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
+{
+    const unsigned short b = __half_as_ushort(__float2half_rn(v * sc));
+    __builtin_nontemporal_store(b, (unsigned short*)p + k);
+}
+
+template <typename S, int LAYOUT>
+__device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
+{
+    const S* __restrict__ fin = (const S*)a.fin;
+    S* __restrict__ fout = (S*)a.fout;
+    const float inv = a.inv_scale;
+
+    int32_t t[Q];
+#pragma unroll
+    for (int j = 1; j < Q; j++) t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+    float d[Q];
+    d[0] = ld_dev(fin, fpos<LAYOUT>(0, i, a.Npad), inv);
+#pragma unroll
+    for (int j = 1; j < Q; j++) {
+        const int64_t src = t[j] >= 0 ? fpos<LAYOUT>(j, t[j], a.Npad) : fpos<LAYOUT>(RE[j], i, a.Npad);
+        d[j] = ld_dev(fin, src, inv);
+    }
+#pragma unroll
+    for (int j = 1; j < Q; j++)
+        if (t[j] == TAB_CONST) d[j] = 0.f;
+
+    float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        drho += d[j];
+        if (EX[j]) jx += EX[j] > 0 ? d[j] : -d[j];
+        if (EY[j]) jy += EY[j] > 0 ? d[j] : -d[j];
+        if (EZ[j]) jz += EZ[j] > 0 ? d[j] : -d[j];
+    }
+    const float rho0 = (float)a.rho0;
+    const float rho = rho0 + drho;
+    if ((rho > 1.2f) | (rho < 0.5f)) record_fault(a, i, (double)rho);
+    const float F = (float)a.gx;
+    if (a.it > 0) jx += 0.5f * F;
+    const float rinv = 1.0f / rho;
+    const float u = jx * rinv, v = jy * rinv, w = jz * rinv;
+    if (a.write_macros) {
+        a.macros[i] = (double)rho;
+        a.macros[a.Npad + i] = (double)u;
+        a.macros[2 * a.Npad + i] = (double)v;
+        a.macros[3 * a.Npad + i] = (double)w;
+    }
+    const float omega = (float)a.omega;
+    const float c1 = 1.0f - omega;
+    const float c2 = (1.0f - 0.5f * omega) * 3.0f * F;
+    const float usq = 1.5f * (u * u + v * v + w * w);
+    const float sc = a.scale;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const float eu = (float)EX[j] * u + (float)EY[j] * v + (float)EZ[j] * w;
+        const float P = eu * (3.0f + 4.5f * eu) - usq;
+        const float wj = (float)TP[j];
+        const float dfeq = wj * (drho + rho * P);
+        const float feq = wj * rho0 + dfeq;
+        const float frc = c2 * ((float)EX[j] - u);
+        const float o = c1 * d[j] + omega * dfeq + frc * feq;
+        st_dev(fout, fpos<LAYOUT>(j, i, a.Npad), o, sc);
+    }
+}
+
+template <typename S, int LAYOUT>
+__device__ __forceinline__ void node_any(const StepArgs& a, int64_t i)
+{
+    if constexpr (sizeof(S) == 8)
+        node_f64<LAYOUT>(a, i);
+    else
+        node_shift<S, LAYOUT>(a, i);
+}
+
+// Single lattice: one thread per node, XCD-contiguous block order.
+template <typename S, int LAYOUT>
+__global__ void __launch_bounds__(1024) k_step(const StepArgs a)
+{
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = blk * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    if (skip_after_fault(a)) return;
+    node_any<S, LAYOUT>(a, i);
+}
+
+// Batch of samples: the grid enumerates the blocks of active samples
+// (active[]), each block lies inside one sample (samples are padded to a
+// multiple of the block size), and the sample supplies omega and gx.
+struct BatchArgs {
+    const int32_t* active;       // global block ids of active samples
+    const int32_t* block_sample; // global block -> sample
+    const int64_t* sample_end;   // first node after each sample's nodes
+    const double* omega;         // per sample
+    const double* gx;            // per sample
+};
+
+template <typename S, int LAYOUT>
+__global__ void __launch_bounds__(1024) k_step_batch(StepArgs a, const BatchArgs b)
+{
+    const int64_t slot = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t gb = b.active[slot];
+    const int s = b.block_sample[gb];
+    const int64_t i = gb * blockDim.x + threadIdx.x;
+    if (i >= b.sample_end[s]) return;
+    if (skip_after_fault(a)) return;
+    a.omega = b.omega[s];
+    a.gx = b.gx[s];
+    node_any<S, LAYOUT>(a, i);
+}
+
+// ---------------------------------------------------------------- init / IO kernels
+template <typename S, int LAYOUT>
+__global__ void k_init_eq(void* f, int64_t N, int64_t Npad, double rho0)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Npad) return;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int64_t k = fpos<LAYOUT>(j, i, Npad);
+        if constexpr (sizeof(S) == 8)
+            ((double*)f)[k] = feq_ref(j, rho0, 0.0, 0.0, 0.0);  // init_dist_macro
+        else if constexpr (sizeof(S) == 4)
+            ((float*)f)[k] = 0.f;
+        else
+            ((__half*)f)[k] = __float2half(0.f);
+    }
+}
+
+// AoS [n][19] FP64 in V order  <->  device layout.  `first` = first V-order
+// node of the chunk, `n` nodes.
+template <typename S, int LAYOUT>
+__global__ void k_read_pop(const void* f, int64_t Npad, const uint32_t* inv_perm, int64_t first,
+                           int64_t n, double rho0, float inv, double* out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int64_t p = fpos<LAYOUT>(j, k, Npad);
+        double v;
+        if constexpr (sizeof(S) == 8)
+            v = ((const double*)f)[p];
+        else if constexpr (sizeof(S) == 4)
+            v = TP[j] * rho0 + (double)((const float*)f)[p];
+        else
+            v = TP[j] * rho0 + (double)(__half2float(((const __half*)f)[p]) * inv);
+        out[r * Q + j] = v;
+    }
+}
+
+template <typename S, int LAYOUT>
+__global__ void k_write_pop(void* f, int64_t Npad, const uint32_t* inv_perm, int64_t first,
+                            int64_t n, double rho0, float sc, const double* in)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int64_t p = fpos<LAYOUT>(j, k, Npad);
+        const double v = in[r * Q + j];
+        if constexpr (sizeof(S) == 8)
+            ((double*)f)[p] = v;
+        else if constexpr (sizeof(S) == 4)
+            ((float*)f)[p] = (float)(v - TP[j] * rho0);
+        else
+            ((__half*)f)[p] = __float2half_rn((float)(v - TP[j] * rho0) * sc);
+    }
+}
+
+__global__ void k_read_macros(const double* macros, int64_t Npad, const uint32_t* inv_perm,
+                              int64_t first, int64_t n, int comp, double* out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    out[r] = macros[comp * Npad + k];
+}
+
+template <int LAYOUT>
+__global__ void k_read_table(const int32_t* tab, int64_t Npad, const uint32_t* inv_perm,
+                             const uint32_t* perm, int64_t first, int64_t n, int64_t N, int32_t* out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+#pragma unroll
+    for (int j = 0; j < Q - 1; j++) {
+        int32_t t = tab[tpos<LAYOUT>(j, k, Npad)];
+        if (t >= 0 && perm) t = (int32_t)perm[t];
+        out[(int64_t)j * n + r] = t;
+    }
+}
+
+// ---------------------------------------------------------------- FP64 reductions
+constexpr int RED_T = 256;
+
+// Block partial sums over nodes [first, first+n): mass (sum of the stored
+// populations; skipped when with_mass == 0) and ux.
+template <typename S, int LAYOUT>
+__global__ void __launch_bounds__(RED_T)
+k_reduce_partial(const void* f, const double* ux, int64_t first, int64_t n, int64_t Npad,
+                 float inv, int with_mass, double* partial)
+{
+    __shared__ double sm[2][RED_T];
+    double m = 0.0, s = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * RED_T + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * RED_T) {
+        const int64_t i = first + r;
+        if (with_mass) {
+            double mi = 0.0;
+#pragma unroll
+            for (int j = 0; j < Q; j++) {
+                const int64_t p = fpos<LAYOUT>(j, i, Npad);
+                if constexpr (sizeof(S) == 8)
+                    mi += ((const double*)f)[p];
+                else if constexpr (sizeof(S) == 4)
+                    mi += (double)((const float*)f)[p];
+                else
+                    mi += (double)(__half2float(((const __half*)f)[p]) * inv);
+            }
+            m += mi;
+        }
+        s += ux[i];
+    }
+    sm[0][threadIdx.x] = m;
+    sm[1][threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_T / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + off];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sm[0][0];
+        partial[2 * blockIdx.x + 1] = sm[1][0];
+    }
+}
+
+// Per-sample sum of ux over each sample's node range (one block per sample).
+__global__ void __launch_bounds__(RED_T)
+k_sample_sums(const double* ux, const int64_t* off, const int64_t* n, double* out)
+{
+    __shared__ double sm[RED_T];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (int64_t r = threadIdx.x; r < n[s]; r += RED_T) acc += ux[off[s] + r];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = RED_T / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = sm[0];
+}
+
+__global__ void __launch_bounds__(RED_T) k_reduce_final(double* partial, int nb)
+{
+    __shared__ double sm[2][RED_T];
+    double m = 0.0, s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_T) {
+        m += partial[2 * b];
+        s += partial[2 * b + 1];
+    }
+    sm[0][threadIdx.x] = m;
+    sm[1][threadIdx.x] = s;
+    __syncthreads();
+    for (int off = RED_T / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + off];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * nb] = sm[0][0];
+        partial[2 * nb + 1] = sm[1][0];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+#define PLB_DISPATCH(PREC, LAYOUT, KERNEL, ...)                                             \
+    do {                                                                                    \
+        if ((PREC) == PLB_FP64) {                                                           \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA>), __VA_ARGS__);        \
+        } else if ((PREC) == PLB_FP32) {                                                    \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA>), __VA_ARGS__);         \
+        } else {                                                                            \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA>), __VA_ARGS__);        \
+        }                                                                                   \
+    } while (0)
+
+static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
+{
+    StepArgs a;
+    a.fin = c->f[c->cur];
+    a.fout = c->f[c->cur ^ 1];
+    a.tab = c->tab;
+    a.N = c->N;
+    a.Npad = c->Npad;
+    a.omega = c->omega;
+    a.gx = c->cfg.gx;
+    a.rho0 = c->cfg.rho0;
+    a.it = it;
+    a.write_macros = write_macros ? 1 : 0;
+    a.macros = c->macros;
+    a.fault = c->fault;
+    a.perm = c->perm;
+    a.scale = c->scale;
+    a.inv_scale = c->inv_scale;
+    return a;
+}
+
+int launch_step(plb_ctx* c, int64_t it, bool write_macros)
+{
+    const StepArgs a = make_args(c, it, write_macros);
+    const int bs = c->cfg.block_size;
+    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_step, dim3(nblk(c->N, bs)), dim3(bs), 0,
+                 c->stream, a);
+    PLB_HIP(hipGetLastError());
+    c->cur ^= 1;
+    return PLB_OK;
+}
+
+// One step over the active blocks of a batch (the context holds the
+// concatenated samples; c->omega / c->cfg.gx are unused).
+int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* active,
+                      int n_active, const int32_t* block_sample, const int64_t* sample_end,
+                      const double* omega, const double* gx)
+{
+    if (n_active > 0) {
+        const StepArgs a = make_args(c, it, write_macros);
+        BatchArgs b;
+        b.active = active;
+        b.block_sample = block_sample;
+        b.sample_end = sample_end;
+        b.omega = omega;
+        b.gx = gx;
+        PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_step_batch, dim3(n_active),
+                     dim3(c->cfg.block_size), 0, c->stream, a, b);
+        PLB_HIP(hipGetLastError());
+    }
+    c->cur ^= 1;
+    return PLB_OK;
+}
+
+
+int launch_init_eq(plb_ctx* c)
+{
+    for (int b = 0; b < 2; b++)
+        PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_init_eq, dim3(nblk(c->Npad, 256)), dim3(256),
+                     0, c->stream, c->f[b], c->N, c->Npad, c->cfg.rho0);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout)
+{
+    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_read_pop, dim3(nblk(n, 256)), dim3(256), 0,
+                 c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, c->cfg.rho0,
+                 c->inv_scale, dout);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din)
+{
+    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_write_pop, dim3(nblk(n, 256)), dim3(256), 0,
+                 c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, c->cfg.rho0, c->scale,
+                 din);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* dout)
+{
+    hipLaunchKernelGGL(k_read_macros, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->macros,
+                       c->Npad, c->inv_perm, first, n, comp, dout);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout)
+{
+    if (c->cfg.layout == PLB_LAYOUT_SOA)
+        hipLaunchKernelGGL(k_read_table<PLB_LAYOUT_SOA>, dim3(nblk(n, 256)), dim3(256), 0,
+                           c->stream, c->tab, c->Npad, c->inv_perm, c->perm, first, n, c->N, dout);
+    else
+        hipLaunchKernelGGL(k_read_table<PLB_LAYOUT_AOSOA>, dim3(nblk(n, 256)), dim3(256), 0,
+                           c->stream, c->tab, c->Npad, c->inv_perm, c->perm, first, n, c->N, dout);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+// out2[0] = sum of stored populations (raw for FP64, deviations otherwise)
+// over nodes [first, first+n) of buffer `buf`, out2[1] = sum of ux macros.
+// Deterministic: fixed grid, fixed tree order.
+int launch_reduce_range(plb_ctx* c, int buf, int64_t first, int64_t n, bool with_mass,
+                        double* out2)
+{
+    const int nb = std::max(1, std::min<int>(c->red_blocks, (int)((n + RED_T - 1) / RED_T)));
+    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_reduce_partial, dim3(nb), dim3(RED_T), 0,
+                 c->stream, c->f[buf], c->macros + c->Npad, first, n, c->Npad, c->inv_scale,
+                 with_mass ? 1 : 0, c->red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_T), 0, c->stream, c->red, nb);
+    PLB_HIP(hipGetLastError());
+    PLB_HIP(hipMemcpyAsync(out2, c->red + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    PLB_HIP(hipStreamSynchronize(c->stream));
+    return PLB_OK;
+}
+
+int launch_reduce(plb_ctx* c, double* out2)
+{
+    return launch_reduce_range(c, c->cur, 0, c->N, true, out2);
+}
+
+int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,
+                       double* d_out)
+{
+    hipLaunchKernelGGL(k_sample_sums, dim3(nsamples), dim3(RED_T), 0, c->stream, c->macros + c->Npad,
+                       d_off, d_n, d_out);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+}  // namespace plb

@@ -1,0 +1,219 @@
+/*
+ * porelb.x -- drop-in for the reference solver (porelb.c:402-443):
+ *
+ *     [mpirun -np P] ./porelb.x it_max
+ *
+ * reads V.bin (every rank's records), runs it_max D3Q19 steps on the GPU
+ * through libplb, and writes part_PPP_TTTTTTT.bin for every rank P of the
+ * V.bin at it % save == 0 and after the last step, like save_data
+ * (porelb.c:375-391, 436, 440).  Messages and exit codes follow the
+ * reference (-10 argc, -1 V.bin, -2 rank count, -3/-4 check_V, -6 density).
+ * Under an MPI launcher (PMI_* or OMPI_COMM_WORLD_* in the environment) the
+ * launcher's size must equal V.bin's num_p; rank 0 drives the GPU for all
+ * ranks and the other ranks exit at once.
+ *
+ * Parameters the reference hard-codes (porelb.c:83-94) come from the
+ * environment; defaults are the reference's values:
+ *   PLB_TAU (0.8)  PLB_GX (1e-3)  PLB_RHO0 (1)  PLB_SAVE (1000)
+ *   PLB_PRECISION fp64|fp32|fp16 (fp64)  PLB_BB refcompat|halfway (refcompat)
+ *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick  PLB_BLOCK (256)
+ *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
+ *   PLB_CHECKPOINT file: populations written after the last step
+ *   PLB_RESTART file: start from such a checkpoint (fills the empty
+ *   save_state/recovery_state stubs of porelb.c:367-373)
+ */
+#define _POSIX_C_SOURCE 199309L
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "plb.h"
+#include "plb_files.h"
+
+static int env_int(const char *k, int def)
+{
+    const char *v = getenv(k);
+    return v && *v ? atoi(v) : def;
+}
+static double env_dbl(const char *k, double def)
+{
+    const char *v = getenv(k);
+    return v && *v ? atof(v) : def;
+}
+static int env_choice(const char *k, const char *a, const char *b, const char *c, int def)
+{
+    const char *v = getenv(k);
+    if (!v || !*v) return def;
+    if (a && !strcmp(v, a)) return 0;
+    if (b && !strcmp(v, b)) return 1;
+    if (c && !strcmp(v, c)) return 2;
+    fprintf(stderr, ">>> bad value %s=%s\n", k, v);
+    exit(-10);
+}
+static double wtime(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+#define CKPT_MAGIC 0x504c4243u /* "PLBC" */
+
+static void save_all(plb_ctx *c, const plb_vfile *v, int it, double *buf)
+{
+    const int64_t N = v->num_node;
+    if (plb_read_macros(c, buf, buf + N, buf + 2 * N, buf + 3 * N)) {
+        fprintf(stderr, "Macro read error!\n");
+        exit(-3);
+    }
+    for (int p = 0; p < v->num_p; p++) {
+        int64_t b = v->V_sub_start[p];
+        int64_t e = p + 1 < v->num_p ? v->V_sub_start[p + 1] : N;
+        if (plb_write_part(p, it, e - b, buf + b, buf + N + b, buf + 2 * N + b, buf + 3 * N + b)) {
+            fprintf(stderr, "Macro binary file open error!\n");
+            exit(-3);
+        }
+    }
+}
+
+int main(int argc, char *argv[])
+{
+    int rank = env_int("PMI_RANK", env_int("OMPI_COMM_WORLD_RANK", 0));
+    int size = env_int("PMI_SIZE", env_int("OMPI_COMM_WORLD_SIZE", -1));
+    if (argc != 2) {
+        fprintf(stderr, ">>> Error argc, usage ./porelb.x it_max\n");
+        exit(-10);
+    }
+    int it_max = atoi(argv[1]);
+
+    plb_vfile v;
+    int rc = plb_read_vfile("V.bin", &v);
+    if (rc == -1) {
+        fprintf(stderr, "Node map file openning error!\n");
+        exit(-1);
+    }
+    if (rc) {
+        fprintf(stderr, "%d >>> Error read V \n", rank);
+        exit(rc);
+    }
+    if (rank == 0) printf("[0]>>> Total num of fluid  nodes = %d\n", v.num_node);
+    if (size > 0 && size != v.num_p) {
+        if (rank == 0)
+            fprintf(stderr, "[0]>>> MPI process num %d differs from num_p %d in V\n", size, v.num_p);
+        exit(-2);
+    }
+    if (rank != 0) return 0;
+    for (int p = 0; p < v.num_p; p++) {
+        int64_t e = p + 1 < v.num_p ? v.V_sub_start[p + 1] : v.num_node;
+        fprintf(stderr, "%d >>> My num of fluid nodes: %d \n", p, (int)(e - v.V_sub_start[p]));
+    }
+
+    plb_config cfg;
+    plb_config_default(&cfg);
+    cfg.tau = env_dbl("PLB_TAU", cfg.tau);
+    cfg.gx = env_dbl("PLB_GX", cfg.gx);
+    cfg.rho0 = env_dbl("PLB_RHO0", cfg.rho0);
+    cfg.precision = env_choice("PLB_PRECISION", "fp64", "fp32", "fp16", PLB_FP64);
+    cfg.bb_mode = env_choice("PLB_BB", "refcompat", "halfway", NULL, PLB_BB_REFCOMPAT);
+    cfg.layout = env_choice("PLB_LAYOUT", "soa", "aosoa", NULL, PLB_LAYOUT_SOA);
+    cfg.ordering = env_choice("PLB_ORDER", "reference", "brick", NULL, PLB_ORDER_REFERENCE);
+    cfg.block_size = env_int("PLB_BLOCK", cfg.block_size);
+    cfg.device = env_int("PLB_DEVICE", 0);
+    const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
+    const int print = env_int("PLB_PRINT", 1);
+    if (save_every < 1) {
+        fprintf(stderr, ">>> PLB_SAVE must be >= 1\n");
+        exit(-10);
+    }
+
+    plb_ctx *c = NULL;
+    rc = plb_create(&c, &cfg, v.V, v.num_node, v.V_sub_start, v.num_p, v.nx, v.ny, v.nz);
+    if (rc == PLB_ERR_BOUNDS) {
+        fprintf(stderr, "%d >>> Check V, bound error!\n", 0);
+        exit(-3);
+    }
+    if (rc == PLB_ERR_PID) {
+        fprintf(stderr, "%d >>> Check V, pe error!\n", 0);
+        exit(-4);
+    }
+    if (rc) {
+        fprintf(stderr, ">>> libplb: %s\n", plb_strerror(rc));
+        exit(rc);
+    }
+    for (int p = 0; p < v.num_p; p++) printf("%d >>> Check V past\n", p);
+
+    const int64_t N = v.num_node;
+    int it0 = 0;
+    const char *restart = getenv("PLB_RESTART");
+    if (restart && *restart) {
+        FILE *fp = fopen(restart, "rb");
+        uint32_t magic = 0;
+        int64_t hdr[2] = {0, 0};
+        double *f = (double *)malloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
+        if (!fp || fread(&magic, 4, 1, fp) != 1 || magic != CKPT_MAGIC ||
+            fread(hdr, 8, 2, fp) != 2 || hdr[1] != N ||
+            fread(f, sizeof(double) * 19, (size_t)N, fp) != (size_t)N) {
+            fprintf(stderr, ">>> cannot read checkpoint %s\n", restart);
+            exit(-5);
+        }
+        fclose(fp);
+        it0 = (int)hdr[0];
+        if (plb_write_populations(c, f)) exit(-5);
+        free(f);
+    }
+
+    double *buf = (double *)malloc(sizeof(double) * 4 * (size_t)(N ? N : 1));
+    double t1 = wtime();
+    int it = it0;
+    while (it < it_max) {
+        /* run up to and including the next save step (or to the end) */
+        int next_save = ((it + save_every - 1) / save_every) * save_every;
+        int last = next_save < it_max ? next_save : it_max - 1;
+        rc = plb_step(c, it, (int64_t)last - it + 1);
+        if (rc == PLB_ERR_DENSITY) {
+            plb_fault ft;
+            plb_last_fault(c, &ft);
+            if (print)
+                for (int s = it; s < ft.step; s++) printf(">>> Step : %20d \n", s);
+            fprintf(stderr, "%d >>> at %d (%d, %d, %d),  Macro denstity  = %g , < 0.5 !\n", ft.pid,
+                    (int)ft.local_id, ft.x, ft.y, ft.z, ft.rho);
+            exit(-6);
+        }
+        if (rc) {
+            fprintf(stderr, ">>> libplb: %s\n", plb_strerror(rc));
+            exit(rc);
+        }
+        if (print)
+            for (int s = it; s <= last; s++) printf(">>> Step : %20d \n", s);
+        if (last % save_every == 0) save_all(c, &v, last, buf);
+        it = last + 1;
+    }
+    double t2 = wtime();
+    printf("Elapsed time is %f\n", t2 - t1);
+    if (t2 > t1 && it_max > it0)
+        printf("MLUPS %.3f (%lld pore nodes x %d steps)\n",
+               (double)N * (it_max - it0) / (t2 - t1) / 1e6, (long long)N, it_max - it0);
+    save_all(c, &v, it_max, buf);
+
+    const char *ckpt = getenv("PLB_CHECKPOINT");
+    if (ckpt && *ckpt) {
+        double *f = (double *)malloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
+        FILE *fp = fopen(ckpt, "wb");
+        uint32_t magic = CKPT_MAGIC;
+        int64_t hdr[2] = {it_max > it0 ? it_max : it0, N};
+        if (!fp || plb_read_populations(c, f)) {
+            fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
+            exit(-3);
+        }
+        fwrite(&magic, 4, 1, fp);
+        fwrite(hdr, 8, 2, fp);
+        fwrite(f, sizeof(double) * 19, (size_t)N, fp);
+        fclose(fp);
+        free(f);
+    }
+    free(buf);
+    plb_destroy(c);
+    plb_free_vfile(&v);
+    return 0;
+}

@@ -1,0 +1,46 @@
+"""Batched samples: every sample of a batch evolves exactly as the same
+sample alone, and per-sample convergence stopping freezes each sample at its
+own step count."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def masks(plb):
+    return [plb.mask_spheres((24, 20, 28), 3.0, 5.0, 0.4, 1),
+            plb.mask_noise((32, 16, 24), 2.0, 0.45, 2),
+            plb.mask_spheres((17, 23, 19), 2.0, 4.0, 0.35, 3)]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_equals_single_runs(plb, prec):
+    fl = masks(plb)
+    taus, gxs = [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6]
+    b = plb.Batch(fl, taus, gxs, precision=prec, bb_mode=plb.BB_HALFWAY)
+    steps, upd, ms = b.run(300, 100, 0.0)   # tol 0: nobody stops early
+    assert list(steps) == [300, 300, 300]
+    assert upd == 300 * sum(b.nodes)
+    red = b.reduce()
+    for s, f in enumerate(fl):
+        with plb.Lattice.from_flag(f, tau=taus[s], gx=gxs[s], precision=prec,
+                                   bb_mode=plb.BB_HALFWAY) as lat:
+            assert lat.N == b.nodes[s]
+            lat.step(0, 300)
+            r = lat.reduce()
+        assert red[s]["sum_ux"] == r["sum_ux"], s
+        assert red[s]["mass"] == pytest.approx(r["mass"], rel=1e-14), s
+
+
+def test_batch_convergence_stops_per_sample(plb):
+    fl = masks(plb)
+    b = plb.Batch(fl, [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6], precision=1, bb_mode=plb.BB_HALFWAY)
+    steps, upd, ms = b.run(20000, 50, 1e-6)
+    assert (steps < 20000).all() and len(set(steps.tolist())) >= 2
+    assert upd == sum(int(s) * n for s, n in zip(steps, b.nodes))
+    red = b.reduce()
+    for s, f in enumerate(fl):
+        with plb.Lattice.from_flag(f, tau=[0.8, 1.0, 0.6][s], gx=[1e-5, 2e-5, 5e-6][s], precision=1,
+                                   bb_mode=plb.BB_HALFWAY) as lat:
+            lat.step(0, int(steps[s]))
+            assert lat.reduce()["sum_ux"] == red[s]["sum_ux"]
