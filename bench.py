@@ -41,6 +41,9 @@ WORKLOADS = {
     "spheres1024-fp16": dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20,
                              seed=4, tau=0.6, gx=5e-7, precision=2, bb=1,
                              label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7"),
+    "spheres1024-fp64": dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20,
+                             seed=4, tau=0.6, gx=5e-7, precision=0, bb=0,
+                             label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7"),
     "spheres256-fp64": dict(kind="spheres", dims=(256, 256, 256), rmin=6.0, rmax=12.0, porosity=0.30,
                             seed=2, tau=0.8, gx=1e-6, precision=0, bb=0,
                             label="256^3 sphere pack, porosity 0.30, tau 0.8, g_x 1e-6"),
@@ -56,14 +59,30 @@ def log(*a):
 
 
 def make_mask(P, w):
+    """Seeded mask; cached per machine under $TMPDIR (a 1024^3 sphere pack
+    takes ~15 s to generate on one host core)."""
+    key = "_".join(str(w[k]) for k in ("kind", "dims", "porosity", "seed") if k in w)
+    key += "_" + "_".join(str(w.get(k)) for k in ("rmin", "rmax", "sigma"))
+    path = os.path.join(tempfile.gettempdir(), "plb_mask_" + "".join(c if c.isalnum() else "_" for c in key) + ".u8")
+    nvox = int(np.prod(w["dims"]))
+    if os.path.exists(path) and os.path.getsize(path) == nvox:
+        return np.fromfile(path, dtype=np.uint8).reshape(w["dims"])
     if w["kind"] == "spheres":
-        return P.mask_spheres(w["dims"], w["rmin"], w["rmax"], w["porosity"], w["seed"])
-    return P.mask_noise(w["dims"], w["sigma"], w["porosity"], w["seed"])
+        flag = P.mask_spheres(w["dims"], w["rmin"], w["rmax"], w["porosity"], w["seed"])
+    else:
+        flag = P.mask_noise(w["dims"], w["sigma"], w["porosity"], w["seed"])
+    try:
+        flag.tofile(path + ".tmp")
+        os.replace(path + ".tmp", path)
+    except OSError:
+        pass
+    return flag
 
 
-def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256):
+def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256, index=0,
+              brick=(16, 4, 4)):
     cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=precision, bb_mode=bb, layout=layout,
-                           ordering=ordering, block_size=block)
+                           ordering=ordering, block_size=block, index_kind=index, brick=brick)
     t0 = time.perf_counter()
     lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
     log(f"lattice built: {lat.N} pore nodes, precision {precision}, {time.perf_counter() - t0:.1f} s")
@@ -148,6 +167,10 @@ def main():
     t0 = time.perf_counter()
     flag = make_mask(P, w)
     log(f"mask {w['dims']} porosity {flag.mean():.4f} in {time.perf_counter() - t0:.1f} s")
+    # Practical ceiling: a streaming copy of 4 GiB (its rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE also calibrate the counter scale for the step kernel).
+    copy_gbs = P.copy_bandwidth(4 << 30, 5)
+    log(f"copy ceiling {copy_gbs:.0f} GB/s")
 
     main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, args.layout,
                          args.ordering, args.block)
@@ -189,7 +212,8 @@ def main():
                    "ordering": args.ordering, "block": args.block},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_update": bpu, "kernel_ms": round(avg_ms, 4)},
+                     "bytes_per_update": bpu, "kernel_ms": round(avg_ms, 4),
+                     "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "secondary": secondary,
         "K": main_run["reduction"]["K"],

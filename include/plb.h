@@ -71,6 +71,11 @@ enum { PLB_LAYOUT_SOA = 0, PLB_LAYOUT_AOSOA = 1 };
  * BRICK = bricks of brick[0] x brick[1] x brick[2] voxels in lexicographic
  * brick order, z,y,x inside a brick.  Outputs are always in V order. */
 enum { PLB_ORDER_REFERENCE = 0, PLB_ORDER_BRICK = 1 };
+/* Pull-table encoding on the device: I32 = one int32 per (node, direction);
+ * TILE8 = per 64-node tile and direction one int32 base plus a u8 offset per
+ * node (254 = constant, 255 = bounce), with int32 entries for the rare
+ * (tile, direction) pairs whose sources span more than 253 ids. */
+enum { PLB_INDEX_I32 = 0, PLB_INDEX_TILE8 = 1 };
 
 typedef struct {
     double tau;          /* relaxation time (reference: 0.8, porelb.c:87)         */
@@ -84,10 +89,11 @@ typedef struct {
     int block_size;      /* threads per block of the step kernel (64..1024)       */
     int device;          /* HIP device ordinal                                    */
     int fp16_scale_log2; /* FP16: stored = (f - w rho0) * 2^k; 0 = derive from gx */
+    int index_kind;      /* PLB_INDEX_I32 | PLB_INDEX_TILE8                       */
 } plb_config;
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
- * REFERENCE order, 256 threads, device 0. */
+ * REFERENCE order/I32 index, 256 threads, device 0. */
 void plb_config_default(plb_config *cfg);
 
 typedef struct plb_ctx plb_ctx;
@@ -176,8 +182,9 @@ typedef struct {
 } plb_fault;
 int plb_last_fault(plb_ctx *c, plb_fault *out);
 
-/* Bytes per node update the step kernel moves as algorithmic traffic
- * (populations read + written, pull-table entries read). */
+/* Bytes per node update the step kernel moves as algorithmic traffic:
+ * populations read + written (2*19*element size) plus the pull-table bytes
+ * of the context's encoding per node. */
 double plb_bytes_per_update(const plb_ctx *c);
 
 const char *plb_strerror(int code);
@@ -191,7 +198,10 @@ int plb_preprocess(const uint8_t *flag, int nx, int ny, int nz, int num_p, int d
 
 /* ---- batched samples: many independent porous samples of uneven pore
  * count advanced by one launch per step over all active samples, with
- * per-sample convergence stopping. */
+ * per-sample convergence stopping.  cfg supplies precision, bounce-back,
+ * layout, index encoding and block size (node order: REFERENCE); tau and gx
+ * come per sample.  With FP16 storage and fp16_scale_log2 == 0 the scale is
+ * derived from the largest |gx| of the samples. */
 typedef struct {
     const uint8_t *flag; /* u8 [nz][ny][nx] */
     int nx, ny, nz;

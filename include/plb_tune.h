@@ -1,0 +1,44 @@
+/*
+ * plb_tune.h -- calibration and autotuning for libplb.  These entry points
+ * have no counterpart in the reference (it has no GPU code); they serve the
+ * benchmark, the command lines and callers that pick a launch shape.
+ */
+#ifndef PLB_TUNE_H
+#define PLB_TUNE_H
+
+#include "plb.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Streaming copy of `bytes` bytes (16-byte vectors, one read and one write
+ * per element) on `device`, best of `iters` launches after one warm-up.
+ * *gbs = 2*bytes / kernel time: the practical HBM ceiling the step kernel is
+ * compared with.  Allocates 2*bytes of device memory for the call. */
+int plb_copy_bandwidth(int device, int64_t bytes, int iters, double *gbs);
+/* bytes must be below 2^39 (one 16-byte element per thread of a 1-D grid). */
+
+/* One timed candidate of the autotuner. */
+typedef struct {
+    plb_config cfg;
+    double mlups;   /* pore-node updates per microsecond over `steps` steps */
+    double ms;      /* device time of `steps` steps */
+} plb_tune_result;
+
+/* Tries layout (SOA, AOSOA) x node order (reference, bricks) x block size
+ * (128, 256, 512) for `base` (precision, bounce-back, index encoding, tau,
+ * gx kept) on the
+ * mask, `steps` timed steps each, and writes the fastest configuration to
+ * *best.  When cache_path is not NULL, a result stored there for the same
+ * mask (64-bit hash of the voxels and dimensions), precision and bounce-back
+ * mode is returned without timing, and new results are appended.
+ * results (may be NULL) receives up to max_results timed candidates. */
+int plb_autotune(const plb_config *base, const uint8_t *flag, int nx, int ny, int nz, int num_p,
+                 int steps, const char *cache_path, plb_config *best, plb_tune_result *results,
+                 int max_results, int *n_results);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libplb.so")
 HEADER = os.path.join(REPO, "include", "plb.h")
+HEADERS = [HEADER, os.path.join(REPO, "include", "plb_tune.h")]
 BIN_DIR = os.path.join(HERE, "bin")
 
 Q = 19
@@ -27,6 +28,7 @@ FP64, FP32, FP16 = 0, 1, 2
 BB_REFCOMPAT, BB_HALFWAY = 0, 1
 LAYOUT_SOA, LAYOUT_AOSOA = 0, 1
 ORDER_REFERENCE, ORDER_BRICK = 0, 1
+INDEX_I32, INDEX_TILE8 = 0, 1
 TYPE_FLUID, TYPE_SOLID, TYPE_EXTERN = 0, 1, 2
 ERR_DENSITY = -6
 
@@ -47,7 +49,7 @@ class Config(C.Structure):
     _fields_ = [("tau", C.c_double), ("gx", C.c_double), ("rho0", C.c_double),
                 ("precision", C.c_int), ("bb_mode", C.c_int), ("layout", C.c_int),
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
-                ("device", C.c_int), ("fp16_scale_log2", C.c_int)]
+                ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int)]
 
 
 class Reduction(C.Structure):
@@ -108,6 +110,8 @@ def _lib():
             "plb_mask_noise": ([p, i32, i32, i32, d, d, C.c_uint64], i32),
             "plb_mask_spheres": ([p, i32, i32, i32, d, d, d, C.c_uint64], i32),
             "plb_mask_bcc": ([p, i32, d], i32),
+            "plb_copy_bandwidth": ([i32, i64, i32, p], i32),
+            "plb_autotune": ([p, p, i32, i32, i32, i32, i32, C.c_char_p, p, p, i32, p], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -121,11 +125,14 @@ def lib():
     return _lib()
 
 
-def header_symbols(path=HEADER):
-    """Names of the functions include/plb.h declares."""
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(plb_[a-z0-9_]+)\s*\(", text)))
+def header_symbols(paths=None):
+    """Names of the functions include/plb.h and include/plb_tune.h declare."""
+    names = set()
+    for path in paths or HEADERS:
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(plb_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def _ptr(a):
@@ -280,6 +287,37 @@ class Lattice:
 
     def bytes_per_update(self):
         return float(_lib().plb_bytes_per_update(self._h))
+
+
+# ------------------------------------------------------------------ tuning
+class TuneResult(C.Structure):
+    _fields_ = [("cfg", Config), ("mlups", C.c_double), ("ms", C.c_double)]
+
+
+def copy_bandwidth(nbytes=8 << 30, iters=5, device=0):
+    """Practical HBM ceiling: GB/s of a streaming copy of nbytes."""
+    g = C.c_double(0.0)
+    _check(_lib().plb_copy_bandwidth(device, int(nbytes), iters, C.byref(g)), "plb_copy_bandwidth")
+    return g.value
+
+
+def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
+    """Fastest (layout, order, block) for this mask; returns (Config, results)."""
+    cfg = cfg or default_config(**kw)
+    flag = np.ascontiguousarray(flag, dtype=np.uint8)
+    nz, ny, nx = flag.shape
+    best = Config()
+    res = (TuneResult * 64)()
+    n = C.c_int(0)
+    _check(_lib().plb_autotune(C.byref(cfg), _ptr(flag), nx, ny, nz, num_p, steps,
+                               cache_path.encode() if cache_path else None, C.byref(best), res, 64,
+                               C.byref(n)), "plb_autotune")
+    out = []
+    for r in res[:n.value]:
+        c = r.cfg
+        out.append(dict(layout=c.layout, ordering=c.ordering, brick=tuple(c.brick), block=c.block_size,
+                        mlups=r.mlups, ms=r.ms))
+    return best, out
 
 
 # ------------------------------------------------------------------ batch

@@ -47,6 +47,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->block_size = 256;
     cfg->device = 0;
     cfg->fp16_scale_log2 = 0;
+    cfg->index_kind = PLB_INDEX_I32;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -85,7 +86,17 @@ int plb::check_cfg(const plb_config* cfg)
         return PLB_ERR_INVALID;
     if (cfg->block_size < 64 || cfg->block_size > 1024 || cfg->block_size % 64) return PLB_ERR_INVALID;
     if (cfg->fp16_scale_log2 < 0 || cfg->fp16_scale_log2 > 100) return PLB_ERR_INVALID;
+    if (cfg->index_kind != PLB_INDEX_I32 && cfg->index_kind != PLB_INDEX_TILE8) return PLB_ERR_INVALID;
     return PLB_OK;
+}
+
+// FP16 storage scale 2^k: deviations of order 1000*|gx| map to ~2048, far
+// from both the FP16 overflow (65504) and its subnormal range (< 6.1e-5).
+int plb::fp16_scale_log2_for(double gx)
+{
+    const double g = fabs(gx) > 0 ? fabs(gx) : 1e-9;
+    const int k = (int)floor(log2(2048.0 / (1000.0 * g)));
+    return std::max(1, std::min(k, 60));
 }
 
 int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, int num_p)
@@ -97,12 +108,7 @@ int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, in
     c->num_p = num_p;
     c->elem = cfg->precision == PLB_FP64 ? 8 : (cfg->precision == PLB_FP32 ? 4 : 2);
     c->omega = 1.0 / cfg->tau;  // porelb.c:88
-    int k = cfg->fp16_scale_log2;
-    if (k == 0) {
-        const double g = fabs(cfg->gx) > 0 ? fabs(cfg->gx) : 1e-9;
-        k = (int)floor(log2(2048.0 / (1000.0 * g)));
-        k = std::max(0, std::min(k, 60));
-    }
+    const int k = cfg->fp16_scale_log2 ? cfg->fp16_scale_log2 : fp16_scale_log2_for(cfg->gx);
     c->scale = cfg->precision == PLB_FP16 ? ldexpf(1.f, k) : 1.f;
     c->inv_scale = cfg->precision == PLB_FP16 ? ldexpf(1.f, -k) : 1.f;
     int ndev = 0;
@@ -140,8 +146,8 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
     }
-    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->perm, c->inv_perm, c->node_xyz,
-                    c->macros, c->fault, c->red};
+    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->twide, c->perm,
+                    c->inv_perm, c->node_xyz, c->macros, c->fault, c->red};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -275,10 +281,14 @@ extern "C" int plb_sync(plb_ctx* c)
     c->pending_n = 0;
     if (key == NO_FAULT) return PLB_OK;
     const int64_t fs = (int64_t)(key >> 32);
-    if (n > 0 && fs >= it0 && fs < it0 + n) {
+    if (n > 0) {
         // Steps after the faulting one did not run: the faulting step's
-        // output buffer holds the current state.
-        c->cur = c->cur_before ^ (int)((fs - it0 + 1) & 1);
+        // output buffer holds the current state.  A fault from an earlier
+        // call made every launch of this one a no-op.
+        if (fs >= it0 && fs < it0 + n)
+            c->cur = c->cur_before ^ (int)((fs - it0 + 1) & 1);
+        else if (fs < it0)
+            c->cur = c->cur_before;
     }
     return PLB_ERR_DENSITY;
 }
@@ -310,7 +320,7 @@ extern "C" int64_t plb_num_nodes(const plb_ctx* c) { return c ? c->N : -1; }
 extern "C" double plb_bytes_per_update(const plb_ctx* c)
 {
     if (!c) return 0.0;
-    return 2.0 * Q * (double)c->elem + (Q - 1) * 4.0;
+    return 2.0 * Q * (double)c->elem + c->index_bytes;
 }
 
 static const int64_t CHUNK_NODES = 1 << 20;

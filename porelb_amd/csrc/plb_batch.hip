@@ -91,6 +91,8 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         if (tabs[s]) (void)hipFree(tabs[s]);
         if (xyzs[s]) (void)hipFree(xyzs[s]);
     }
+    c->index_bytes = (Q - 1) * 4.0;
+    if (!rc && cfg->index_kind == PLB_INDEX_TILE8) rc = compress_tile8(c);
     return rc;
 }
 
@@ -114,8 +116,15 @@ extern "C" int plb_batch_create(plb_batch** out, const plb_config* cfg, const pl
     b->off.assign(n_samples, 0);
     b->n.assign(n_samples, 0);
     b->c = new plb_ctx();
-    rc = ctx_begin(b->c, cfg, samples[0].nx, samples[0].ny, samples[0].nz, 1);
-    if (!rc) rc = batch_build(b, cfg, samples);
+    plb_config bcfg = *cfg;
+    if (bcfg.precision == PLB_FP16 && bcfg.fp16_scale_log2 == 0) {
+        // one storage scale for all samples: the largest body force decides
+        double gmax = 0.0;
+        for (int s = 0; s < n_samples; s++) gmax = std::max(gmax, fabs(samples[s].gx));
+        bcfg.fp16_scale_log2 = fp16_scale_log2_for(gmax);
+    }
+    rc = ctx_begin(b->c, &bcfg, samples[0].nx, samples[0].ny, samples[0].nz, 1);
+    if (!rc) rc = batch_build(b, &bcfg, samples);
     if (!rc) rc = ctx_alloc_state(b->c);
     if (!rc) {
         const int bs = cfg->block_size;

@@ -429,6 +429,129 @@ int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref)
         PLB_HIP(hipGetLastError());
     }
     PLB_HIP(hipStreamSynchronize(s));
+    c->index_bytes = (Q - 1) * 4.0;
+    if (c->cfg.index_kind == PLB_INDEX_TILE8) return compress_tile8(c);
+    return PLB_OK;
+}
+
+// ---------------------------------------------------------------- TILE8 index
+// One 64-thread block per tile: wave-wide min/max of the stream sources of
+// each direction decide between a u8 offset from a base and a wide (int32)
+// slot.
+constexpr int TILE8_SPAN = 253;
+
+__device__ __forceinline__ int32_t wave_min(int32_t v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t wave_max(int32_t v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void tile_span(const int32_t* tab, int64_t Npad, int64_t N, int64_t n,
+                                          int jm1, int32_t& t, int32_t& mn, int32_t& mx)
+{
+    t = n < N ? tab[tpos<LAYOUT>(jm1, n, Npad)] : TAB_CONST;
+    mn = wave_min(t >= 0 ? t : INT32_MAX);
+    mx = wave_max(t >= 0 ? t : -1);
+}
+
+template <int LAYOUT>
+__global__ void __launch_bounds__(64)
+k_tile8_flags(const int32_t* __restrict__ tab, int64_t Npad, int64_t N, uint64_t* __restrict__ wide)
+{
+    const int64_t tile = blockIdx.x;
+    const int64_t n = tile * 64 + threadIdx.x;
+    for (int jm1 = 0; jm1 < Q - 1; jm1++) {
+        int32_t t, mn, mx;
+        tile_span<LAYOUT>(tab, Npad, N, n, jm1, t, mn, mx);
+        if (threadIdx.x == 0) wide[tile * (Q - 1) + jm1] = (mx >= 0 && mx - mn > TILE8_SPAN) ? 1 : 0;
+    }
+}
+
+// Offsets are packed four directions per 32-bit word: word k of a lane holds
+// directions 4k+1..4k+4 in bytes 0..3 ([tile][5][64] words, 20 bytes per
+// node), so a lane reads its 18 offsets with 5 loads.
+template <int LAYOUT>
+__global__ void __launch_bounds__(64)
+k_tile8_write(const int32_t* __restrict__ tab, int64_t Npad, int64_t N,
+              const uint64_t* __restrict__ wide, const uint64_t* __restrict__ slot,
+              int32_t* __restrict__ desc, uint32_t* __restrict__ off, int32_t* __restrict__ wtab)
+{
+    const int64_t tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t n = tile * 64 + lane;
+    uint32_t words[TILE8_WORDS];
+#pragma unroll
+    for (int k = 0; k < TILE8_WORDS; k++) words[k] = 0xfefefefeu;
+    for (int jm1 = 0; jm1 < Q - 1; jm1++) {
+        int32_t t, mn, mx;
+        tile_span<LAYOUT>(tab, Npad, N, n, jm1, t, mn, mx);
+        const int64_t k = tile * (Q - 1) + jm1;
+        uint32_t o;
+        if (wide[k]) {
+            wtab[(int64_t)slot[k] * 64 + lane] = t;
+            if (lane == 0) desc[k] = -(int32_t)slot[k] - 1;
+            o = 254;
+        } else {
+            const int32_t base = mx >= 0 ? mn : 0;
+            if (lane == 0) desc[k] = base;
+            o = t >= 0 ? (uint32_t)(t - base) : (t == TAB_BOUNCE ? 255u : 254u);
+        }
+        const int sh = 8 * (jm1 & 3);
+        words[jm1 >> 2] = (words[jm1 >> 2] & ~(0xffu << sh)) | (o << sh);
+    }
+#pragma unroll
+    for (int k = 0; k < TILE8_WORDS; k++) off[(tile * TILE8_WORDS + k) * 64 + lane] = words[k];
+}
+
+// Replaces the int32 table of a context by the TILE8 encoding.
+int compress_tile8(plb_ctx* c)
+{
+    hipStream_t s = c->stream;
+    const int64_t T = c->Npad / 64;
+    const int64_t nk = T * (Q - 1);
+    uint64_t *wide = nullptr, *slot = nullptr;
+    PLB_HIP(hipMalloc(&wide, sizeof(uint64_t) * nk));
+    PLB_HIP(hipMalloc(&slot, sizeof(uint64_t) * nk));
+    if (c->cfg.layout == PLB_LAYOUT_SOA)
+        hipLaunchKernelGGL(k_tile8_flags<PLB_LAYOUT_SOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
+                           c->Npad, c->N, wide);
+    else
+        hipLaunchKernelGGL(k_tile8_flags<PLB_LAYOUT_AOSOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
+                           c->Npad, c->N, wide);
+    PLB_HIP(hipGetLastError());
+    int rc = exclusive_scan_u64(wide, slot, nk, s);
+    if (rc) return rc;
+    uint64_t last_slot = 0, last_wide = 0;
+    PLB_HIP(hipMemcpyAsync(&last_slot, slot + nk - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&last_wide, wide + nk - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipStreamSynchronize(s));
+    c->n_wide = (int64_t)(last_slot + last_wide);
+    PLB_HIP(hipMalloc(&c->tdesc, sizeof(int32_t) * nk));
+    PLB_HIP(hipMalloc(&c->toff, sizeof(uint32_t) * T * TILE8_WORDS * 64));
+    PLB_HIP(hipMalloc(&c->twide, sizeof(int32_t) * 64 * (c->n_wide ? c->n_wide : 1)));
+    if (c->cfg.layout == PLB_LAYOUT_SOA)
+        hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_SOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
+                           c->Npad, c->N, wide, slot, c->tdesc, c->toff, c->twide);
+    else
+        hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_AOSOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
+                           c->Npad, c->N, wide, slot, c->tdesc, c->toff, c->twide);
+    PLB_HIP(hipGetLastError());
+    PLB_HIP(hipStreamSynchronize(s));
+    (void)hipFree(wide);
+    (void)hipFree(slot);
+    (void)hipFree(c->tab);
+    c->tab = nullptr;
+    c->index_bytes = c->N ? ((double)T * TILE8_WORDS * 256 + (double)nk * 4 + (double)c->n_wide * 256) /
+                                (double)c->N
+                          : 0.0;
     return PLB_OK;
 }
 

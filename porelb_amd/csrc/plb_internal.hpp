@@ -4,6 +4,10 @@
 //   populations  f[2]   two ping-pong buffers of Npad*19 storage elements
 //                       (double, or float/half holding f - w_j*rho0)
 //   pull table   tab    Npad*18 int32: >=0 source node, -1 bounce, -2 const
+//                or TILE8: tdesc [T][18] int32 (base >= 0, or -(wide slot+1)),
+//                toff [T][5][64] u32, four u8 offsets per word (0..253
+//                offset, 254 const, 255 bounce),
+//                twide [W][64] int32 entries of the wide (tile, direction) pairs
 //   macros              [4][Npad] double (rho, ux, uy, uz), written on
 //                       request (last step of a call, convergence checks)
 //   perm/inv_perm       internal id <-> V-order id, only for BRICK order
@@ -35,6 +39,7 @@ constexpr double TP[Q] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 
 constexpr int32_t TAB_BOUNCE = -1;
 constexpr int32_t TAB_CONST = -2;
 constexpr unsigned long long NO_FAULT = ~0ull;
+constexpr int TILE8_WORDS = 5;  // u32 words of packed u8 offsets per node
 
 // Position of population (dir j, node n).
 template <int LAYOUT>
@@ -80,7 +85,12 @@ struct plb_ctx {
     size_t elem = 8;            // bytes per stored population
     void* f[2] = {nullptr, nullptr};
     int cur = 0;                // f[cur] holds the current post-collision state
-    int32_t* tab = nullptr;     // internal layout
+    int32_t* tab = nullptr;     // internal layout (I32 index)
+    int32_t* tdesc = nullptr;   // TILE8 index
+    uint32_t* toff = nullptr;
+    int32_t* twide = nullptr;
+    int64_t n_wide = 0;
+    double index_bytes = 72.0;  // pull-table bytes per node
     uint32_t* perm = nullptr;   // internal -> V order (null: identity)
     uint32_t* inv_perm = nullptr;
     uint64_t* node_xyz = nullptr;  // internal order
@@ -129,6 +139,8 @@ int launch_reduce_range(plb_ctx* c, int buf, int64_t first, int64_t n, bool with
 int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,
                        double* d_out);
 int check_cfg(const plb_config* cfg);
+int fp16_scale_log2_for(double gx);
+int compress_tile8(plb_ctx* c);
 int ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, int num_p);
 int ctx_alloc_state(plb_ctx* c);
 }  // namespace plb

@@ -30,6 +30,9 @@ struct StepArgs {
     const void* fin;
     void* fout;
     const int32_t* tab;
+    const int32_t* tdesc;
+    const uint32_t* toff;
+    const int32_t* twide;
     int64_t N, Npad;
     double omega, gx, rho0;
     int64_t it;
@@ -67,17 +70,56 @@ __device__ __forceinline__ bool skip_after_fault(const StepArgs& a)
     return fk != NO_FAULT && (int64_t)(fk >> 32) < a.it;
 }
 
+// The 18 pull-table entries of node i (t[1..18]).  TILE8: the wave is one
+// 64-node tile, so its 18 descriptors are uniform (scalar loads); each lane
+// then reads one byte per direction, or an int32 for a wide pair.
+template <int LAYOUT, int IDX>
+__device__ __forceinline__ void load_entries(const StepArgs& a, int64_t i, int32_t t[Q])
+{
+    if constexpr (IDX == PLB_INDEX_I32) {
+#pragma unroll
+        for (int j = 1; j < Q; j++)
+            t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+    } else {
+        const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
+        const int lane = (int)(i & 63);
+        const int32_t* d = a.tdesc + tile * (Q - 1);
+        const uint32_t* o = a.toff + tile * TILE8_WORDS * 64 + lane;
+        uint32_t w[TILE8_WORDS];
+#pragma unroll
+        for (int k = 0; k < TILE8_WORDS; k++) w[k] = __builtin_nontemporal_load(o + k * 64);
+        int32_t desc[Q];
+        int32_t any = 0;
+#pragma unroll
+        for (int j = 1; j < Q; j++) {
+            desc[j] = d[j - 1];
+            any |= desc[j];
+        }
+#pragma unroll
+        for (int j = 1; j < Q; j++) {
+            // one select: 0..253 -> base + v, 254 -> -2 (const), 255 -> -1 (bounce)
+            const int32_t v = (int32_t)((w[(j - 1) >> 2] >> (8 * ((j - 1) & 3))) & 0xffu);
+            t[j] = v < 254 ? desc[j] + v : v - 256;
+        }
+        if (any < 0) {  // a (tile, direction) pair of this tile is wide
+#pragma unroll
+            for (int j = 1; j < Q; j++)
+                if (desc[j] < 0)
+                    t[j] = __builtin_nontemporal_load(a.twide + (int64_t)(-desc[j] - 1) * 64 + lane);
+        }
+    }
+}
+
 // One node of the FP64 step, reference arithmetic.  The caller has checked
 // i < N.  a.omega / a.gx are the node's sample parameters.
-template <int LAYOUT>
+template <int LAYOUT, int IDX>
 __device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i)
 {
     const double* __restrict__ fin = (const double*)a.fin;
     double* __restrict__ fout = (double*)a.fout;
 
     int32_t t[Q];
-#pragma unroll
-    for (int j = 1; j < Q; j++) t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+    load_entries<LAYOUT, IDX>(a, i, t);
     double f[Q];
     f[0] = fin[fpos<LAYOUT>(0, i, a.Npad)];
 #pragma unroll
@@ -133,7 +175,7 @@ __device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
     __builtin_nontemporal_store(b, (unsigned short*)p + k);
 }
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int IDX>
 __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
 {
     const S* __restrict__ fin = (const S*)a.fin;
@@ -141,8 +183,7 @@ __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
     const float inv = a.inv_scale;
 
     int32_t t[Q];
-#pragma unroll
-    for (int j = 1; j < Q; j++) t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+    load_entries<LAYOUT, IDX>(a, i, t);
     float d[Q];
     d[0] = ld_dev(fin, fpos<LAYOUT>(0, i, a.Npad), inv);
 #pragma unroll
@@ -193,24 +234,24 @@ __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
     }
 }
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int IDX>
 __device__ __forceinline__ void node_any(const StepArgs& a, int64_t i)
 {
     if constexpr (sizeof(S) == 8)
-        node_f64<LAYOUT>(a, i);
+        node_f64<LAYOUT, IDX>(a, i);
     else
-        node_shift<S, LAYOUT>(a, i);
+        node_shift<S, LAYOUT, IDX>(a, i);
 }
 
 // Single lattice: one thread per node, XCD-contiguous block order.
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int IDX>
 __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
 {
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
     if (skip_after_fault(a)) return;
-    node_any<S, LAYOUT>(a, i);
+    node_any<S, LAYOUT, IDX>(a, i);
 }
 
 // Batch of samples: the grid enumerates the blocks of active samples
@@ -224,7 +265,7 @@ struct BatchArgs {
     const double* gx;            // per sample
 };
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int IDX>
 __global__ void __launch_bounds__(1024) k_step_batch(StepArgs a, const BatchArgs b)
 {
     const int64_t slot = xcd_block(blockIdx.x, gridDim.x);
@@ -235,7 +276,7 @@ __global__ void __launch_bounds__(1024) k_step_batch(StepArgs a, const BatchArgs
     if (skip_after_fault(a)) return;
     a.omega = b.omega[s];
     a.gx = b.gx[s];
-    node_any<S, LAYOUT>(a, i);
+    node_any<S, LAYOUT, IDX>(a, i);
 }
 
 // ---------------------------------------------------------------- init / IO kernels
@@ -311,19 +352,37 @@ __global__ void k_read_macros(const double* macros, int64_t Npad, const uint32_t
     out[r] = macros[comp * Npad + k];
 }
 
-template <int LAYOUT>
-__global__ void k_read_table(const int32_t* tab, int64_t Npad, const uint32_t* inv_perm,
-                             const uint32_t* perm, int64_t first, int64_t n, int64_t N, int32_t* out)
+template <int LAYOUT, int IDX>
+__global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const uint32_t* perm,
+                             int64_t first, int64_t n, int32_t* out)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t g = first + r;
     const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
-#pragma unroll
-    for (int j = 0; j < Q - 1; j++) {
-        int32_t t = tab[tpos<LAYOUT>(j, k, Npad)];
-        if (t >= 0 && perm) t = (int32_t)perm[t];
-        out[(int64_t)j * n + r] = t;
+    int32_t t[Q];
+    if constexpr (IDX == PLB_INDEX_I32) {
+        load_entries<LAYOUT, IDX>(a, k, t);
+    } else {
+        // Lanes of a wave hold unrelated nodes here: decode without the
+        // per-wave descriptor shortcut.
+        const int64_t tile = k >> 6;
+        const int lane = (int)(k & 63);
+        for (int j = 1; j < Q; j++) {
+            const int32_t desc = a.tdesc[tile * (Q - 1) + j - 1];
+            if (desc >= 0) {
+                const uint32_t word = a.toff[(tile * TILE8_WORDS + ((j - 1) >> 2)) * 64 + lane];
+                const int32_t v = (int32_t)((word >> (8 * ((j - 1) & 3))) & 0xffu);
+                t[j] = v == 255 ? TAB_BOUNCE : (v == 254 ? TAB_CONST : desc + v);
+            } else {
+                t[j] = a.twide[(int64_t)(-desc - 1) * 64 + lane];
+            }
+        }
+    }
+    for (int j = 1; j < Q; j++) {
+        int32_t v = t[j];
+        if (v >= 0 && perm) v = (int32_t)perm[v];
+        out[(int64_t)(j - 1) * n + r] = v;
     }
 }
 
@@ -432,12 +491,42 @@ static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
         }                                                                                   \
     } while (0)
 
+#define PLB_DISPATCH3(PREC, LAYOUT, IDX, KERNEL, ...)                                        \
+    do {                                                                                     \
+        if ((IDX) == PLB_INDEX_I32) {                                                        \
+            if ((PREC) == PLB_FP64) {                                                        \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
+            } else if ((PREC) == PLB_FP32) {                                                 \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
+            } else {                                                                         \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
+            }                                                                                \
+        } else {                                                                             \
+            if ((PREC) == PLB_FP64) {                                                        \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+            } else if ((PREC) == PLB_FP32) {                                                 \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+            } else {                                                                         \
+                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
+            }                                                                                \
+        }                                                                                    \
+    } while (0)
+
 static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
 {
     StepArgs a;
     a.fin = c->f[c->cur];
     a.fout = c->f[c->cur ^ 1];
     a.tab = c->tab;
+    a.tdesc = c->tdesc;
+    a.toff = c->toff;
+    a.twide = c->twide;
     a.N = c->N;
     a.Npad = c->Npad;
     a.omega = c->omega;
@@ -457,8 +546,8 @@ int launch_step(plb_ctx* c, int64_t it, bool write_macros)
 {
     const StepArgs a = make_args(c, it, write_macros);
     const int bs = c->cfg.block_size;
-    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_step, dim3(nblk(c->N, bs)), dim3(bs), 0,
-                 c->stream, a);
+    PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step, dim3(nblk(c->N, bs)),
+                  dim3(bs), 0, c->stream, a);
     PLB_HIP(hipGetLastError());
     c->cur ^= 1;
     return PLB_OK;
@@ -478,8 +567,8 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
         b.sample_end = sample_end;
         b.omega = omega;
         b.gx = gx;
-        PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_step_batch, dim3(n_active),
-                     dim3(c->cfg.block_size), 0, c->stream, a, b);
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step_batch,
+                      dim3(n_active), dim3(c->cfg.block_size), 0, c->stream, a, b);
         PLB_HIP(hipGetLastError());
     }
     c->cur ^= 1;
@@ -524,12 +613,24 @@ int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* d
 
 int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout)
 {
-    if (c->cfg.layout == PLB_LAYOUT_SOA)
-        hipLaunchKernelGGL(k_read_table<PLB_LAYOUT_SOA>, dim3(nblk(n, 256)), dim3(256), 0,
-                           c->stream, c->tab, c->Npad, c->inv_perm, c->perm, first, n, c->N, dout);
-    else
-        hipLaunchKernelGGL(k_read_table<PLB_LAYOUT_AOSOA>, dim3(nblk(n, 256)), dim3(256), 0,
-                           c->stream, c->tab, c->Npad, c->inv_perm, c->perm, first, n, c->N, dout);
+    const StepArgs a = make_args(c, 0, false);
+    const dim3 g(nblk(n, 256)), b(256);
+    const bool t8 = c->cfg.index_kind == PLB_INDEX_TILE8;
+    if (c->cfg.layout == PLB_LAYOUT_SOA) {
+        if (t8)
+            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), g, b, 0, c->stream, a,
+                               c->inv_perm, c->perm, first, n, dout);
+        else
+            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_SOA, PLB_INDEX_I32>), g, b, 0, c->stream, a,
+                               c->inv_perm, c->perm, first, n, dout);
+    } else {
+        if (t8)
+            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), g, b, 0, c->stream,
+                               a, c->inv_perm, c->perm, first, n, dout);
+        else
+            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), g, b, 0, c->stream, a,
+                               c->inv_perm, c->perm, first, n, dout);
+    }
     PLB_HIP(hipGetLastError());
     return PLB_OK;
 }

@@ -1,0 +1,168 @@
+// plb_tune.hip -- HBM copy ceiling and the launch-shape autotuner.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "plb_internal.hpp"
+#include "plb_tune.h"
+
+using namespace plb;
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One 16-byte element per thread, non-temporal both ways: the fastest plain
+// copy form measured on MI355X (grid-stride and unrolled forms are slower).
+__global__ void __launch_bounds__(256) k_copy16(const u32x4* __restrict__ in, u32x4* __restrict__ out,
+                                                int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
+}
+
+extern "C" int plb_copy_bandwidth(int device, int64_t bytes, int iters, double* gbs)
+{
+    if (!gbs || bytes < 16 * 1024 || bytes >= ((int64_t)1 << 39) || iters < 1) return PLB_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PLB_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return PLB_ERR_INVALID;
+    PLB_HIP(hipSetDevice(device));
+    const int64_t n = bytes / 16;
+    u32x4 *a = nullptr, *b = nullptr;
+    PLB_HIP(hipMalloc(&a, n * 16));
+    PLB_HIP(hipMalloc(&b, n * 16));
+    PLB_HIP(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    PLB_HIP(hipEventCreate(&e0));
+    PLB_HIP(hipEventCreate(&e1));
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    float best = 1e30f;
+    for (int k = 0; k <= iters; k++) {
+        PLB_HIP(hipEventRecord(e0, 0));
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, 0, a, b, n);
+        PLB_HIP(hipEventRecord(e1, 0));
+        PLB_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        PLB_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (k > 0) best = std::min(best, t);
+    }
+    *gbs = 2.0 * (double)(n * 16) / (best * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return PLB_OK;
+}
+
+static uint64_t mask_hash(const uint8_t* flag, int64_t n, int nx, int ny, int nz)
+{
+    // FNV-1a over the voxels and the dimensions.
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    mix((uint64_t)nx);
+    mix((uint64_t)ny);
+    mix((uint64_t)nz);
+    const uint64_t* w = (const uint64_t*)flag;
+    const int64_t nw = n / 8;
+    for (int64_t i = 0; i < nw; i++) {
+        uint64_t v;
+        memcpy(&v, w + i, 8);
+        mix(v);
+    }
+    for (int64_t i = nw * 8; i < n; i++) mix(flag[i]);
+    return h;
+}
+
+static bool cache_lookup(const char* path, uint64_t key, int prec, int bb, plb_config* best)
+{
+    FILE* fp = fopen(path, "r");
+    if (!fp) return false;
+    char line[512];
+    bool found = false;
+    while (fgets(line, sizeof line, fp)) {
+        unsigned long long k;
+        int p, b, layout, order, bx, by, bz, bs;
+        if (sscanf(line, "%llx %d %d %d %d %d %d %d %d", &k, &p, &b, &layout, &order, &bx, &by, &bz,
+                   &bs) == 9 &&
+            k == key && p == prec && b == bb) {
+            best->layout = layout;
+            best->ordering = order;
+            best->brick[0] = bx;
+            best->brick[1] = by;
+            best->brick[2] = bz;
+            best->block_size = bs;
+            found = true;  // the last matching line wins
+        }
+    }
+    fclose(fp);
+    return found;
+}
+
+extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx, int ny, int nz,
+                            int num_p, int steps, const char* cache_path, plb_config* best,
+                            plb_tune_result* results, int max_results, int* n_results)
+{
+    if (!base || !flag || !best || steps < 1) return PLB_ERR_INVALID;
+    int rc = check_cfg(base);
+    if (rc) return rc;
+    if (n_results) *n_results = 0;
+    *best = *base;
+    const int64_t nvox = (int64_t)nx * ny * nz;
+    const uint64_t key = mask_hash(flag, nvox, nx, ny, nz);
+    if (cache_path && cache_lookup(cache_path, key, base->precision, base->bb_mode, best))
+        return PLB_OK;
+
+    struct Cand {
+        int layout, order, bx, by, bz, bs;
+    };
+    std::vector<Cand> cands;
+    const int bricks[][3] = {{0, 0, 0}, {16, 4, 4}, {32, 4, 2}, {8, 8, 8}, {64, 2, 2}};
+    for (int layout = 0; layout < 2; layout++)
+        for (const auto& b : bricks)
+            for (int bs : {128, 256, 512})
+                cands.push_back({layout, b[0] ? 1 : 0, b[0] ? b[0] : base->brick[0],
+                                 b[0] ? b[1] : base->brick[1], b[0] ? b[2] : base->brick[2], bs});
+    double best_mlups = -1.0;
+    int nres = 0;
+    for (const Cand& cd : cands) {
+        plb_config cfg = *base;
+        cfg.layout = cd.layout;
+        cfg.ordering = cd.order;
+        cfg.brick[0] = cd.bx;
+        cfg.brick[1] = cd.by;
+        cfg.brick[2] = cd.bz;
+        cfg.block_size = cd.bs;
+        plb_ctx* c = nullptr;
+        rc = plb_create_from_flag(&c, &cfg, flag, nx, ny, nz, num_p);
+        if (rc) return rc;
+        double ms = 0.0;
+        rc = plb_step(c, 0, std::max(2, steps / 4));
+        if (!rc) rc = plb_step_timed(c, 0, steps, &ms);
+        const double n = (double)plb_num_nodes(c);
+        plb_destroy(c);
+        if (rc) return rc;
+        const double mlups = n * steps / (ms * 1e-3) / 1e6;
+        if (results && nres < max_results) results[nres] = {cfg, mlups, ms};
+        nres++;
+        if (mlups > best_mlups) {
+            best_mlups = mlups;
+            *best = cfg;
+        }
+    }
+    if (n_results) *n_results = std::min(nres, max_results);
+    if (cache_path) {
+        FILE* fp = fopen(cache_path, "a");
+        if (fp) {
+            fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
+                    base->precision, base->bb_mode, best->layout, best->ordering, best->brick[0],
+                    best->brick[1], best->brick[2], best->block_size, best_mlups);
+            fclose(fp);
+        }
+    }
+    return PLB_OK;
+}

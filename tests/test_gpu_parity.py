@@ -69,19 +69,29 @@ def test_table_from_V_and_flag_bitexact(plb, mode, np_):
         assert np.array_equal(lat.read_table(), want)
     for layout, order in [(plb.LAYOUT_AOSOA, plb.ORDER_REFERENCE), (plb.LAYOUT_SOA, plb.ORDER_BRICK),
                           (plb.LAYOUT_AOSOA, plb.ORDER_BRICK)]:
-        with plb.Lattice.from_flag(flag, np_, bb_mode=mode, layout=layout, ordering=order,
-                                   brick=(4, 2, 3)) as lat:
-            assert np.array_equal(lat.read_table(), want), (layout, order)
+        for index in (plb.INDEX_I32, plb.INDEX_TILE8):
+            with plb.Lattice.from_flag(flag, np_, bb_mode=mode, layout=layout, ordering=order,
+                                       brick=(4, 2, 3), index_kind=index) as lat:
+                assert np.array_equal(lat.read_table(), want), (layout, order, index)
 
 
 def test_table_large_bitexact(plb):
-    flag = plb.mask_spheres((160, 128, 96), 6.0, 12.0, 0.3, 11)
+    """Wide rows (x-wraps span more than a byte of ids) exercise the TILE8
+    int32 fallback."""
+    flag = plb.mask_spheres((64, 160, 640), 6.0, 12.0, 0.3, 11)
     say("table large")
     V, start = O.build_V(flag, 4)
     for mode in (O.BB_REFCOMPAT, O.BB_HALFWAY):
         want = O.pull_table(V, start, mode)
         with plb.Lattice.from_flag(flag, 4, bb_mode=mode) as lat:
             assert np.array_equal(lat.read_table(), want)
+            b32 = lat.bytes_per_update()
+        for layout in (0, 1):
+            with plb.Lattice.from_flag(flag, 4, bb_mode=mode, index_kind=plb.INDEX_TILE8,
+                                       layout=layout) as lat:
+                assert np.array_equal(lat.read_table(), want)
+                b8 = lat.bytes_per_update()
+        assert b32 == 2 * 19 * 8 + 72 and 2 * 19 * 8 + 18 < b8 < 2 * 19 * 8 + 24
 
 
 # ---------------------------------------------------------------- FP64 step parity
@@ -127,7 +137,9 @@ def test_fp64_layouts_orders_blocks_identical(plb, mode):
     f0, _ = O.init_equilibrium(len(V))
     mac = O.pull_run(tab, f0, 60, 0.7, 1e-4)
     variants = [dict(), dict(layout=1), dict(ordering=1), dict(layout=1, ordering=1, brick=(8, 8, 8)),
-                dict(block_size=64), dict(block_size=1024), dict(ordering=1, brick=(40, 1, 1))]
+                dict(block_size=64), dict(block_size=1024), dict(ordering=1, brick=(40, 1, 1)),
+                dict(index_kind=1), dict(index_kind=1, layout=1, block_size=512),
+                dict(index_kind=1, layout=1, ordering=1, brick=(8, 8, 8))]
     for kw in variants:
         say("variant", mode, kw)
         with plb.Lattice.from_flag(flag, 2, tau=0.7, gx=1e-4, bb_mode=mode, **kw) as lat:
