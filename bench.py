@@ -33,24 +33,26 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 
+# Launch shapes come from plb_autotune on each workload's mask (MI355X):
+# layout 1 = AoSoA tiles of 64 nodes, index 1 = TILE8 pull table.
+SPHERES1024 = dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20, seed=4,
+                   tau=0.6, gx=5e-7, label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7")
 WORKLOADS = {
-    # name: mask kind, dims (nz,ny,nx), mask params, tau, gx, precision, bb
-    "spheres1024-fp32": dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20,
-                             seed=4, tau=0.6, gx=5e-7, precision=1, bb=1,
-                             label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7"),
-    "spheres1024-fp16": dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20,
-                             seed=4, tau=0.6, gx=5e-7, precision=2, bb=1,
-                             label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7"),
-    "spheres1024-fp64": dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20,
-                             seed=4, tau=0.6, gx=5e-7, precision=0, bb=0,
-                             label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7"),
+    "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=0, block=128),
+    "spheres1024-fp16": dict(SPHERES1024, precision=2, bb=1, index=1, layout=1, ordering=0, block=128),
+    "spheres1024-fp64": dict(SPHERES1024, precision=0, bb=0, index=0, layout=1, ordering=1,
+                             brick=(8, 8, 8), block=256),
     "spheres256-fp64": dict(kind="spheres", dims=(256, 256, 256), rmin=6.0, rmax=12.0, porosity=0.30,
-                            seed=2, tau=0.8, gx=1e-6, precision=0, bb=0,
+                            seed=2, tau=0.8, gx=1e-6, precision=0, bb=0, index=0, layout=1, ordering=1,
+                            brick=(8, 8, 8), block=256,
                             label="256^3 sphere pack, porosity 0.30, tau 0.8, g_x 1e-6"),
     "noise512-fp32": dict(kind="noise", dims=(512, 512, 512), sigma=3.0, porosity=0.25, seed=3,
-                          tau=1.0, gx=1e-6, precision=1, bb=1,
+                          tau=1.0, gx=1e-6, precision=1, bb=1, index=1, layout=1, ordering=0, block=128,
                           label="512^3 smoothed noise sigma 3, porosity 0.25, tau 1.0, g_x 1e-6"),
 }
+# The plain FP64 path the reduced-precision paths are compared with: the
+# reference bounce order and the int32 pull table, in its fastest launch shape.
+FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=1, brick=(8, 8, 8), block=256)
 PREC_NAME = {0: "f64", 1: "f32-store/f32-collide", 2: "f16-store/f32-collide"}
 
 
@@ -138,6 +140,89 @@ def cpu_baseline(P, w, max_seconds=20.0):
             "kind": "port", "sample": f"{dims[0]}^3 domain, {len(V)} pore nodes, {steps} FP64 steps, 1 thread"}
 
 
+def study_precision(P, max_steps, check, tol):
+    """Config 3: 512^3 smoothed noise (sigma 3, porosity 0.25), tau 1.0,
+    g_x 1e-6, half-way bounce-back; FP64 vs FP32-storage vs FP16-storage run
+    to steady state (|d<u_x>| <= tol |<u_x>| between checks).  Reports the
+    relative L2 error of the velocity field and of K against FP64."""
+    w = WORKLOADS["noise512-fp32"]
+    flag = make_mask(P, w)
+    runs = {}
+    for prec in (0, 1, 2):
+        cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
+                               block_size=128)
+        with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
+            m0 = lat.reduce()["mass"]
+            t0 = time.perf_counter()
+            prev, it = None, 0
+            while it < max_steps:
+                lat.step(it, check)
+                it += check
+                r = lat.reduce()
+                if prev is not None and abs(r["mean_ux"] - prev) <= tol * abs(r["mean_ux"]):
+                    break
+                prev = r["mean_ux"]
+            wall = time.perf_counter() - t0
+            mac = lat.read_macros()
+            runs[prec] = dict(steps=it, K=r["K"], mean_ux=r["mean_ux"], mass_drift=(r["mass"] - m0) / m0,
+                              wall_s=wall, mlups=lat.N * it / wall / 1e6, u=mac[1:].copy(), nodes=lat.N,
+                              converged=it < max_steps)
+        log(f"precision {prec}: {runs[prec]['steps']} steps, K {runs[prec]['K']:.9e}")
+    u64 = runs[0]["u"]
+    out = {"study": "precision", "workload": w["label"], "pore_nodes": runs[0]["nodes"], "tol": tol,
+           "check_every": check, "results": {}}
+    for prec, name in ((0, "fp64"), (1, "fp32_store"), (2, "fp16_store")):
+        r = runs[prec]
+        out["results"][name] = {
+            "steps": r["steps"], "converged": r["converged"], "K": r["K"], "mean_ux": r["mean_ux"],
+            "mass_rel_drift": r["mass_drift"], "mlups_incl_checks": round(r["mlups"], 1),
+            "velocity_rel_l2_err": float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64)),
+            "K_rel_err": abs(r["K"] / runs[0]["K"] - 1.0)}
+    print(json.dumps(out), flush=True)
+
+
+def study_batch(P, n_samples, max_steps, check, tol, seed=100):
+    """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
+    sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
+    TILE8, tau 1.0, g_x 1e-6.  One batched launch per step over all active
+    samples (per-sample convergence stop) vs the same samples run one after
+    another (one sample per launch, same kernels and convergence rule)."""
+    rng = np.random.default_rng(seed)
+    flags = []
+    for k in range(n_samples):
+        L = int(rng.integers(64, 193))
+        phi = float(rng.uniform(0.15, 0.45))
+        if k % 2 == 0:
+            flags.append(P.mask_spheres((L, L, L), 6.0, 12.0, phi, seed + k))
+        else:
+            flags.append(P.mask_noise((L, L, L), 2.0, phi, seed + k))
+    taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
+    kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, block_size=128)
+    b = P.Batch(flags, taus, gxs, **kw)
+    steps, upd, ms = b.run(max_steps, check, tol)
+    red = b.reduce()
+    nodes = list(b.nodes)
+    b.close()
+    seq_upd, seq_ms, seq_steps = 0, 0.0, []
+    for k, f in enumerate(flags):
+        one = P.Batch([f], [taus[k]], [gxs[k]], **kw)
+        st, u, m = one.run(max_steps, check, tol)
+        seq_upd += u
+        seq_ms += m
+        seq_steps.append(int(st[0]))
+        one.close()
+    out = {"study": "batch", "samples": n_samples, "pore_nodes_total": int(sum(nodes)),
+           "tol": tol, "check_every": check, "max_steps": max_steps,
+           "batched": {"updates": int(upd), "ms": round(ms, 1), "mlups": round(upd / (ms * 1e-3) / 1e6, 1)},
+           "sequential": {"updates": int(seq_upd), "ms": round(seq_ms, 1),
+                          "mlups": round(seq_upd / (seq_ms * 1e-3) / 1e6, 1)},
+           "steps_min": int(min(steps)), "steps_max": int(max(steps)),
+           "same_steps_as_sequential": [int(a) for a in steps] == seq_steps,
+           "K_range": [float(min(r["K"] for r in red)), float(max(r["K"] for r in red))]}
+    out["speedup"] = round(out["batched"]["mlups"] / out["sequential"]["mlups"], 3)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,9 +231,16 @@ def main():
     ap.add_argument("--workload", default="spheres1024-fp32", choices=sorted(WORKLOADS))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--layout", type=int, default=0)
-    ap.add_argument("--ordering", type=int, default=0)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--layout", type=int, default=None)
+    ap.add_argument("--ordering", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--index", type=int, default=None)
+    ap.add_argument("--study", choices=["precision", "batch"], default=None,
+                    help="run a study instead of the benchmark line")
+    ap.add_argument("--max-steps", type=int, default=100000)
+    ap.add_argument("--check", type=int, default=200)
+    ap.add_argument("--tol", type=float, default=None)
+    ap.add_argument("--samples", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -157,6 +249,12 @@ def main():
 
     import porelb_amd as P
 
+    if args.study == "precision":
+        study_precision(P, args.max_steps, args.check, args.tol if args.tol is not None else 1e-8)
+        return
+    if args.study == "batch":
+        study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6)
+        return
     w = WORKLOADS[args.workload]
     cpu = None
     if not args.skip_cpu:
@@ -172,27 +270,34 @@ def main():
     copy_gbs = P.copy_bandwidth(4 << 30, 5)
     log(f"copy ceiling {copy_gbs:.0f} GB/s")
 
-    main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, args.layout,
-                         args.ordering, args.block)
+    shape = {k: (getattr(args, k) if getattr(args, k) is not None else w[k])
+             for k in ("layout", "ordering", "block", "index")}
+    main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, shape["layout"],
+                         shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)))
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
     avg_ms = main_run["event_ms"] / steps
     bpu = main_run["bytes_per_update"]
     achieved = bpu * n / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+    tfile = os.path.join(REPO, "profiles", f"step_{args.workload}_pmc.md")
+    if os.path.exists(tfile):   # written from two rocprofv3 --pmc passes
+        for line in open(tfile):
+            if line.startswith("hbm_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
 
     secondary = {}
     if not args.skip_fp64 and w["precision"] != 0:
-        fp64 = time_path(P, flag, w, 0, 0, max(20, steps // 2), max(5, args.warmup // 2))
+        fp = FP64_PLAIN
+        fp64 = time_path(P, flag, w, 0, 0, max(20, steps // 2), max(5, args.warmup // 2), fp["layout"],
+                         fp["ordering"], fp["block"], fp["index"], fp["brick"])
         fp64_mlups = fp64["nodes"] * max(20, steps // 2) / fp64["wall_s"] / 1e6
         secondary = {"fp64_plain_mlups": round(fp64_mlups, 1),
                      "fp64_plain_roofline_frac": round(fp64["bytes_per_update"] * fp64["nodes"] /
                                                        (fp64["event_ms"] / max(20, steps // 2) * 1e-3) /
                                                        1e9 / HBM_PEAK_GBS, 4),
-                     "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3)}
+                     "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3),
+                     "fp64_plain_shape": "int32 table, AoSoA, 8^3 bricks, 256 threads"}
 
     out = {
         "metric": "pore-node MLUPS",
@@ -208,10 +313,13 @@ def main():
         "dtype": "f64" if w["precision"] == 0 else "f32",
         "data": "synthetic (seeded periodic mask)",
         "config": {"workload": w["label"], "pore_nodes": n, "storage": PREC_NAME[w["precision"]],
-                   "bounce_back": "halfway" if w["bb"] else "reference order", "layout": args.layout,
-                   "ordering": args.ordering, "block": args.block},
+                   "bounce_back": "halfway" if w["bb"] else "reference order",
+                   "pull_table": "TILE8" if shape["index"] else "int32",
+                   "layout": "AoSoA64" if shape["layout"] else "SoA",
+                   "node_order": "bricks" if shape["ordering"] else "reference", "block": shape["block"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "mlups_vs_228B_roofline": round(mlups / (HBM_PEAK_GBS * 1e9 / 228.0 / 1e6), 4),
                      "bytes_per_update": bpu, "kernel_ms": round(avg_ms, 4),
                      "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,

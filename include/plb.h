@@ -210,9 +210,10 @@ typedef struct {
 typedef struct plb_batch plb_batch;
 int plb_batch_create(plb_batch **out, const plb_config *cfg, const plb_sample *samples,
                      int n_samples);
-/* Runs until every sample has converged (|d<u_x>|/|<u_x>| < tol between
- * checks check_every steps apart) or max_steps.  steps_done[s] = steps the
- * sample ran; updates[0] = total node updates performed. */
+/* Runs until every sample has converged (|d<u_x>| <= tol * max(|<u_x>|,
+ * 1e-3 |g_x|) between checks check_every steps apart; the floor stops samples
+ * without a percolating path) or max_steps.  steps_done[s] = steps the
+ * sample ran; updates[0] = total node updates performed; ms = device time. */
 int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
                   int64_t *steps_done, int64_t *updates, double *ms);
 /* Per-sample reductions after a run (n_samples entries). */

@@ -18,7 +18,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libplb.so")
+# PLB_LIBRARY points at another build of the same library (e.g. a variant
+# built for timing experiments); default: the in-tree build.
+LIB_PATH = os.environ.get("PLB_LIBRARY") or os.path.join(HERE, "libplb.so")
 HEADER = os.path.join(REPO, "include", "plb.h")
 HEADERS = [HEADER, os.path.join(REPO, "include", "plb_tune.h")]
 BIN_DIR = os.path.join(HERE, "bin")

@@ -250,7 +250,10 @@ extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_ever
         for (int s = 0; s < b->S; s++) {
             if (!b->active[s]) continue;
             const double mean = sums[s] / (double)b->n[s];
-            if (!isnan(prev[s]) && fabs(mean - prev[s]) <= tol * fabs(mean)) {
+            // relative change, with a floor of 1e-3 |g_x| on the scale so a
+            // sample without a percolating path (mean u_x -> 0) also stops
+            const double scale = fmax(fabs(mean), 1e-3 * fabs(b->gx[s]));
+            if (!isnan(prev[s]) && fabs(mean - prev[s]) <= tol * scale) {
                 b->active[s] = 0;
                 b->sample_cur[s] = c->cur;
                 changed = true;

@@ -3,7 +3,7 @@
 // reductions.
 //
 // One kernel per time step, one thread per pore node:
-//   1. 18 pull-table entries (int32, streamed with non-temporal loads);
+//   1. 18 pull-table entries (int32, or TILE8 bases + packed u8 offsets);
 //   2. 19 population gathers issued back to back (stream: f[j][src];
 //      bounce: f[re[j]][self]; const: no data needed);
 //   3. collision with body force (porelb.c:318-365);
@@ -62,12 +62,13 @@ __device__ __forceinline__ void record_fault(const StepArgs& a, int64_t i, doubl
     a.macros[i] = rho;  // rho of the offending node, read back by plb_last_fault
 }
 
-__device__ __forceinline__ bool skip_after_fault(const StepArgs& a)
+// A step after the first faulting step stores nothing, so the state stays
+// that of the faulting step (the reference exits there).  The fault word is
+// loaded at kernel entry but only consulted before the stores, so it adds no
+// latency in front of the table and population loads.
+__device__ __forceinline__ bool frozen(unsigned long long fk, int64_t it)
 {
-    // A step after the first faulting step does nothing, so the state stays
-    // that of the faulting step (the reference exits there).
-    const unsigned long long fk = *a.fault;
-    return fk != NO_FAULT && (int64_t)(fk >> 32) < a.it;
+    return fk != NO_FAULT && (int64_t)(fk >> 32) < it;
 }
 
 // The 18 pull-table entries of node i (t[1..18]).  TILE8: the wave is one
@@ -79,7 +80,7 @@ __device__ __forceinline__ void load_entries(const StepArgs& a, int64_t i, int32
     if constexpr (IDX == PLB_INDEX_I32) {
 #pragma unroll
         for (int j = 1; j < Q; j++)
-            t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
+            t[j] = a.tab[tpos<LAYOUT>(j - 1, i, a.Npad)];
     } else {
         const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
         const int lane = (int)(i & 63);
@@ -87,7 +88,7 @@ __device__ __forceinline__ void load_entries(const StepArgs& a, int64_t i, int32
         const uint32_t* o = a.toff + tile * TILE8_WORDS * 64 + lane;
         uint32_t w[TILE8_WORDS];
 #pragma unroll
-        for (int k = 0; k < TILE8_WORDS; k++) w[k] = __builtin_nontemporal_load(o + k * 64);
+        for (int k = 0; k < TILE8_WORDS; k++) w[k] = o[k * 64];
         int32_t desc[Q];
         int32_t any = 0;
 #pragma unroll
@@ -113,7 +114,7 @@ __device__ __forceinline__ void load_entries(const StepArgs& a, int64_t i, int32
 // One node of the FP64 step, reference arithmetic.  The caller has checked
 // i < N.  a.omega / a.gx are the node's sample parameters.
 template <int LAYOUT, int IDX>
-__device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i)
+__device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i, unsigned long long fk)
 {
     const double* __restrict__ fin = (const double*)a.fin;
     double* __restrict__ fout = (double*)a.fout;
@@ -140,6 +141,7 @@ __device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i)
         rhov += EY[j] * f[j];
         rhow += EZ[j] * f[j];
     }
+    if (frozen(fk, a.it)) return;
     if ((rho > 1.2) | (rho < 0.5)) record_fault(a, i, rho);
     if (a.it > 0) rhou += 0.5 * a.gx;
     const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
@@ -176,7 +178,7 @@ __device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
 }
 
 template <typename S, int LAYOUT, int IDX>
-__device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
+__device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i, unsigned long long fk)
 {
     const S* __restrict__ fin = (const S*)a.fin;
     S* __restrict__ fout = (S*)a.fout;
@@ -205,6 +207,7 @@ __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
     }
     const float rho0 = (float)a.rho0;
     const float rho = rho0 + drho;
+    if (frozen(fk, a.it)) return;
     if ((rho > 1.2f) | (rho < 0.5f)) record_fault(a, i, (double)rho);
     const float F = (float)a.gx;
     if (a.it > 0) jx += 0.5f * F;
@@ -235,12 +238,12 @@ __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i)
 }
 
 template <typename S, int LAYOUT, int IDX>
-__device__ __forceinline__ void node_any(const StepArgs& a, int64_t i)
+__device__ __forceinline__ void node_any(const StepArgs& a, int64_t i, unsigned long long fk)
 {
     if constexpr (sizeof(S) == 8)
-        node_f64<LAYOUT, IDX>(a, i);
+        node_f64<LAYOUT, IDX>(a, i, fk);
     else
-        node_shift<S, LAYOUT, IDX>(a, i);
+        node_shift<S, LAYOUT, IDX>(a, i, fk);
 }
 
 // Single lattice: one thread per node, XCD-contiguous block order.
@@ -250,8 +253,7 @@ __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
-    if (skip_after_fault(a)) return;
-    node_any<S, LAYOUT, IDX>(a, i);
+    node_any<S, LAYOUT, IDX>(a, i, *a.fault);
 }
 
 // Batch of samples: the grid enumerates the blocks of active samples
@@ -273,10 +275,9 @@ __global__ void __launch_bounds__(1024) k_step_batch(StepArgs a, const BatchArgs
     const int s = b.block_sample[gb];
     const int64_t i = gb * blockDim.x + threadIdx.x;
     if (i >= b.sample_end[s]) return;
-    if (skip_after_fault(a)) return;
     a.omega = b.omega[s];
     a.gx = b.gx[s];
-    node_any<S, LAYOUT, IDX>(a, i);
+    node_any<S, LAYOUT, IDX>(a, i, *a.fault);
 }
 
 // ---------------------------------------------------------------- init / IO kernels

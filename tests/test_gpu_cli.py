@@ -1,0 +1,111 @@
+"""The drop-in command lines on the GPU: preprocess.x + porelb.x reproduce
+the reference's V.bin fields and part files bit for bit, keep its density
+guard message and exit code, resume from a checkpoint, and postprocess.x
+writes the reference's slice format."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, REPO, load_tiny
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(REPO, "porelb_amd", "bin")
+
+
+def run(args, cwd, env=None, timeout=120):
+    e = dict(os.environ)
+    e.update(env or {})
+    print("[case]", " ".join(args), env, flush=True)
+    return subprocess.run(args, cwd=cwd, capture_output=True, text=True, env=e, timeout=timeout)
+
+
+def read_parts(cwd, num_p, it):
+    parts = [np.fromfile(os.path.join(cwd, f"part_{p:03d}_{it:07d}.bin")).reshape(4, -1) for p in range(num_p)]
+    return np.concatenate(parts, axis=1)
+
+
+@pytest.mark.parametrize("name", ["tiny_a", "tiny_d"])
+def test_preprocess_and_porelb_match_reference(tmp_path, name):
+    d = load_tiny(name)
+    flag = d["flag"]
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    for np_ in d["ranks"]:
+        r = run([os.path.join(BIN, "preprocess.x"), str(np_), str(nx), str(ny), str(nz)], tmp_path)
+        assert r.returncode == 0, r.stderr
+        vb = O.read_vbin(str(tmp_path / "V.bin"))
+        x, y, z, t, pid, nid = O.semantic_fields(vb["V"])
+        assert np.array_equal(vb["start"], d[f"np{np_}_start"])
+        for k, v in dict(x=x, y=y, z=z, type=t, pid=pid, id=nid).items():
+            assert np.array_equal(v, d[f"np{np_}_V_{k}"].astype(np.int64)), k
+        steps = int(d["steps"])
+        env = {"PLB_TAU": repr(float(d["tau"])), "PLB_GX": repr(float(d["gx"])), "PLB_PRINT": "0"}
+        r = run([os.path.join(BIN, "porelb.x"), str(steps)], tmp_path, env)
+        assert r.returncode == 0, r.stderr
+        assert np.array_equal(read_parts(tmp_path, int(np_), steps), d[f"np{np_}_macros"])
+        assert os.path.exists(tmp_path / f"part_{int(np_) - 1:03d}_0000000.bin")  # it % 1000 == 0 save
+
+
+def test_porelb_under_mpirun(tmp_path):
+    """`mpirun -np P ./porelb.x` with V.bin for P ranks: one GPU process does
+    the work for every rank's records."""
+    if not os.path.exists(O.MPIRUN):
+        pytest.skip("no mpirun in this image")
+    d = load_tiny("tiny_a")
+    flag = d["flag"]
+    V, start = O.build_V(flag, 2)
+    import porelb_amd as P
+
+    P.write_vbin(str(tmp_path / "V.bin"), V, start, flag.shape)
+    env = {"PLB_TAU": repr(float(d["tau"])), "PLB_GX": repr(float(d["gx"])), "PLB_PRINT": "0"}
+    r = run([O.MPIRUN, "-np", "2", os.path.join(BIN, "porelb.x"), str(int(d["steps"]))], tmp_path, env)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(read_parts(tmp_path, 2, int(d["steps"])), d["np2_macros"])
+    r = run([O.MPIRUN, "-np", "3", os.path.join(BIN, "porelb.x"), "5"], tmp_path, env)
+    assert r.returncode != 0 and "differs from num_p 2" in r.stderr
+
+
+def test_density_guard_cli(tmp_path):
+    d = dict(np.load(os.path.join(GOLDEN, "fault.npz")))
+    flag = d["flag"]
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "1", "8", "8", "8"], tmp_path).returncode == 0
+    r = run([os.path.join(BIN, "porelb.x"), "200"], tmp_path,
+            {"PLB_TAU": repr(float(d["tau"])), "PLB_GX": repr(float(d["gx"])), "PLB_PRINT": "0"})
+    assert r.returncode == int(d["returncode"]) == 250
+    assert str(d["message"]) in r.stderr
+
+
+def test_checkpoint_resume_and_postprocess(tmp_path):
+    flag = (np.random.default_rng(9).random((10, 9, 12)) < 0.6).astype(np.uint8)
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "1", "12", "9", "10"], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-4", "PLB_PRINT": "0", "PLB_SAVE": "25"}
+    straight = tmp_path / "straight"
+    straight.mkdir()
+    os.link(tmp_path / "V.bin", straight / "V.bin")
+    assert run([os.path.join(BIN, "porelb.x"), "50"], straight, env).returncode == 0
+    r = run([os.path.join(BIN, "porelb.x"), "30"], tmp_path, dict(env, PLB_CHECKPOINT="ck.bin"))
+    assert r.returncode == 0, r.stderr
+    r = run([os.path.join(BIN, "porelb.x"), "50"], tmp_path, dict(env, PLB_RESTART="ck.bin"))
+    assert r.returncode == 0, r.stderr
+    a = np.fromfile(straight / "part_000_0000050.bin")
+    b = np.fromfile(tmp_path / "part_000_0000050.bin")
+    assert np.array_equal(a, b)
+    # postprocess: X slice at x = 3 of step 50 against the part file scattered by hand
+    r = run([os.path.join(BIN, "postprocess.x"), "X", "3", "50"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    lines = open(tmp_path / "Slice_X_0000050.dat").read().splitlines()
+    assert lines[1].startswith("Variables") and "I =    9, J =   10" in lines[2]
+    vals = np.array([[float(t) for t in ln.split()] for ln in lines[3:]])
+    V = O.read_vbin(str(tmp_path / "V.bin"))["V"]
+    mac = b.reshape(4, -1)
+    grid = np.zeros((4, 10, 9, 12))
+    grid[:, V["z"], V["y"], V["x"]] = mac
+    want_rho = grid[0, :, :, 3].reshape(-1)
+    assert np.allclose(vals[:, 2], want_rho, rtol=1e-13, atol=0)
+    assert np.allclose(vals[:, 3], grid[1, :, :, 3].reshape(-1), rtol=1e-13, atol=1e-300)
+    assert np.array_equal(vals[:, 6], flag[:, :, 3].reshape(-1).astype(float))
