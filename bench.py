@@ -33,26 +33,30 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 
-# Launch shapes come from plb_autotune on each workload's mask (MI355X):
-# layout 1 = AoSoA tiles of 64 nodes, index 1 = TILE8 pull table.
+# Launch shapes measured on MI355X (plb_autotune candidates and a finer
+# sweep): layout 1 = AoSoA tiles of 64 nodes; index 1 = TILE8 pull table;
+# ordering 2 = pencils of `brick` = (nx, Py, 1) rows visited plane by plane.
 SPHERES1024 = dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20, seed=4,
                    tau=0.6, gx=5e-7, label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7")
 WORKLOADS = {
-    "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=0, block=128),
-    "spheres1024-fp16": dict(SPHERES1024, precision=2, bb=1, index=1, layout=1, ordering=0, block=128),
-    "spheres1024-fp64": dict(SPHERES1024, precision=0, bb=0, index=0, layout=1, ordering=1,
-                             brick=(8, 8, 8), block=256),
+    "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=2,
+                             brick=(1024, 128, 1), block=256),
+    "spheres1024-fp16": dict(SPHERES1024, precision=2, bb=1, index=1, layout=1, ordering=2,
+                             brick=(1024, 64, 1), block=128),
+    "spheres1024-fp64": dict(SPHERES1024, precision=0, bb=0, index=0, layout=1, ordering=2,
+                             brick=(1024, 64, 1), block=256),
     "spheres256-fp64": dict(kind="spheres", dims=(256, 256, 256), rmin=6.0, rmax=12.0, porosity=0.30,
-                            seed=2, tau=0.8, gx=1e-6, precision=0, bb=0, index=0, layout=1, ordering=1,
-                            brick=(8, 8, 8), block=256,
+                            seed=2, tau=0.8, gx=1e-6, precision=0, bb=0, index=0, layout=1, ordering=2,
+                            brick=(256, 64, 1), block=256,
                             label="256^3 sphere pack, porosity 0.30, tau 0.8, g_x 1e-6"),
     "noise512-fp32": dict(kind="noise", dims=(512, 512, 512), sigma=3.0, porosity=0.25, seed=3,
-                          tau=1.0, gx=1e-6, precision=1, bb=1, index=1, layout=1, ordering=0, block=128,
+                          tau=1.0, gx=1e-6, precision=1, bb=1, index=1, layout=1, ordering=2,
+                          brick=(512, 64, 1), block=256,
                           label="512^3 smoothed noise sigma 3, porosity 0.25, tau 1.0, g_x 1e-6"),
 }
 # The plain FP64 path the reduced-precision paths are compared with: the
 # reference bounce order and the int32 pull table, in its fastest launch shape.
-FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=1, brick=(8, 8, 8), block=256)
+FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=2, brick=(1024, 64, 1), block=256)
 PREC_NAME = {0: "f64", 1: "f32-store/f32-collide", 2: "f16-store/f32-collide"}
 
 
@@ -147,10 +151,11 @@ def study_precision(P, max_steps, check, tol):
     relative L2 error of the velocity field and of K against FP64."""
     w = WORKLOADS["noise512-fp32"]
     flag = make_mask(P, w)
+    log(f"mask {w['dims']} porosity {flag.mean():.4f}")
     runs = {}
     for prec in (0, 1, 2):
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
-                               block_size=128)
+                               ordering=2, brick=w["brick"], block_size=256)
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
             t0 = time.perf_counter()
@@ -159,6 +164,8 @@ def study_precision(P, max_steps, check, tol):
                 lat.step(it, check)
                 it += check
                 r = lat.reduce()
+                if (it // check) % 20 == 0:
+                    log(f"precision {prec}: step {it}, <u_x> {r['mean_ux']:.12e}")
                 if prev is not None and abs(r["mean_ux"] - prev) <= tol * abs(r["mean_ux"]):
                     break
                 prev = r["mean_ux"]
@@ -197,7 +204,7 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100):
         else:
             flags.append(P.mask_noise((L, L, L), 2.0, phi, seed + k))
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
-    kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, block_size=128)
+    kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256)
     b = P.Batch(flags, taus, gxs, **kw)
     steps, upd, ms = b.run(max_steps, check, tol)
     red = b.reduce()
@@ -289,15 +296,16 @@ def main():
     secondary = {}
     if not args.skip_fp64 and w["precision"] != 0:
         fp = FP64_PLAIN
+        brick = (w["dims"][2],) + fp["brick"][1:]
         fp64 = time_path(P, flag, w, 0, 0, max(20, steps // 2), max(5, args.warmup // 2), fp["layout"],
-                         fp["ordering"], fp["block"], fp["index"], fp["brick"])
+                         fp["ordering"], fp["block"], fp["index"], brick)
         fp64_mlups = fp64["nodes"] * max(20, steps // 2) / fp64["wall_s"] / 1e6
         secondary = {"fp64_plain_mlups": round(fp64_mlups, 1),
                      "fp64_plain_roofline_frac": round(fp64["bytes_per_update"] * fp64["nodes"] /
                                                        (fp64["event_ms"] / max(20, steps // 2) * 1e-3) /
                                                        1e9 / HBM_PEAK_GBS, 4),
                      "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3),
-                     "fp64_plain_shape": "int32 table, AoSoA, 8^3 bricks, 256 threads"}
+                     "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads"}
 
     out = {
         "metric": "pore-node MLUPS",
@@ -316,7 +324,8 @@ def main():
                    "bounce_back": "halfway" if w["bb"] else "reference order",
                    "pull_table": "TILE8" if shape["index"] else "int32",
                    "layout": "AoSoA64" if shape["layout"] else "SoA",
-                   "node_order": "bricks" if shape["ordering"] else "reference", "block": shape["block"]},
+                   "node_order": ["reference", "bricks", "pencils"][shape["ordering"]],
+                   "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "mlups_vs_228B_roofline": round(mlups / (HBM_PEAK_GBS * 1e9 / 228.0 / 1e6), 4),

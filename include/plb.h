@@ -69,8 +69,11 @@ enum { PLB_BB_REFCOMPAT = 0, PLB_BB_HALFWAY = 1 };
 enum { PLB_LAYOUT_SOA = 0, PLB_LAYOUT_AOSOA = 1 };
 /* Device node order: REFERENCE = V order (z,y,x lexicographic per rank);
  * BRICK = bricks of brick[0] x brick[1] x brick[2] voxels in lexicographic
- * brick order, z,y,x inside a brick.  Outputs are always in V order. */
-enum { PLB_ORDER_REFERENCE = 0, PLB_ORDER_BRICK = 1 };
+ * brick order (x fastest), z,y,x inside a brick; PENCIL = the same bricks
+ * visited z fastest, then y, then x, so with bricks of (nx, Py, 1) the nodes
+ * run through pencils of Py rows plane by plane and a node's z-neighbours are
+ * Py rows (not a whole plane) away.  Outputs are always in V order. */
+enum { PLB_ORDER_REFERENCE = 0, PLB_ORDER_BRICK = 1, PLB_ORDER_PENCIL = 2 };
 /* Pull-table encoding on the device: I32 = one int32 per (node, direction);
  * TILE8 = per 64-node tile and direction one int32 base plus a u8 offset per
  * node (254 = constant, 255 = bounce), with int32 entries for the rare
@@ -199,8 +202,8 @@ int plb_preprocess(const uint8_t *flag, int nx, int ny, int nz, int num_p, int d
 /* ---- batched samples: many independent porous samples of uneven pore
  * count advanced by one launch per step over all active samples, with
  * per-sample convergence stopping.  cfg supplies precision, bounce-back,
- * layout, index encoding and block size (node order: REFERENCE); tau and gx
- * come per sample.  With FP16 storage and fp16_scale_log2 == 0 the scale is
+ * layout, index encoding, node order (PENCIL uses each sample's own x extent
+ * as the pencil width) and block size; tau and gx come per sample.  With FP16 storage and fp16_scale_log2 == 0 the scale is
  * derived from the largest |gx| of the samples. */
 typedef struct {
     const uint8_t *flag; /* u8 [nz][ny][nx] */

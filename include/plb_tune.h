@@ -26,8 +26,8 @@ typedef struct {
     double ms;      /* device time of `steps` steps */
 } plb_tune_result;
 
-/* Tries layout (SOA, AOSOA) x node order (reference, bricks) x block size
- * (128, 256, 512) for `base` (precision, bounce-back, index encoding, tau,
+/* Tries layout (SOA, AOSOA) x node order (reference; bricks 16x4x4 and 8^3;
+ * pencils of 16, 64, 128 full rows) x block size (128, 256, 512) for `base` (precision, bounce-back, index encoding, tau,
  * gx kept) on the
  * mask, `steps` timed steps each, and writes the fastest configuration to
  * *best.  When cache_path is not NULL, a result stored there for the same

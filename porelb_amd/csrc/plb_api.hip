@@ -78,11 +78,12 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->precision < PLB_FP64 || cfg->precision > PLB_FP16) return PLB_ERR_INVALID;
     if (cfg->bb_mode != PLB_BB_REFCOMPAT && cfg->bb_mode != PLB_BB_HALFWAY) return PLB_ERR_INVALID;
     if (cfg->layout != PLB_LAYOUT_SOA && cfg->layout != PLB_LAYOUT_AOSOA) return PLB_ERR_INVALID;
-    if (cfg->ordering != PLB_ORDER_REFERENCE && cfg->ordering != PLB_ORDER_BRICK)
+    if (cfg->ordering != PLB_ORDER_REFERENCE && cfg->ordering != PLB_ORDER_BRICK &&
+        cfg->ordering != PLB_ORDER_PENCIL)
         return PLB_ERR_INVALID;
-    if (cfg->ordering == PLB_ORDER_BRICK &&
+    if (cfg->ordering != PLB_ORDER_REFERENCE &&
         (cfg->brick[0] < 1 || cfg->brick[1] < 1 || cfg->brick[2] < 1 ||
-         cfg->brick[0] > 1024 || cfg->brick[1] > 1024 || cfg->brick[2] > 1024))
+         cfg->brick[0] > 65535 || cfg->brick[1] > 65535 || cfg->brick[2] > 65535))
         return PLB_ERR_INVALID;
     if (cfg->block_size < 64 || cfg->block_size > 1024 || cfg->block_size % 64) return PLB_ERR_INVALID;
     if (cfg->fp16_scale_log2 < 0 || cfg->fp16_scale_log2 > 100) return PLB_ERR_INVALID;

@@ -66,6 +66,27 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         std::vector<int32_t> start;
         rc = build_ref_from_flag(smp[s].flag, smp[s].nx, smp[s].ny, smp[s].nz, 1, cfg->bb_mode,
                                  c->stream, &tabs[s], &xyzs[s], &ns, &start);
+        if (!rc && cfg->ordering != PLB_ORDER_REFERENCE && ns > 0) {
+            // The sample's node order (bricks or pencils with the sample's own
+            // x extent for pencils).  Batch outputs are per-sample sums, so
+            // the permutation itself is not kept.
+            plb_config scfg = *cfg;
+            if (scfg.ordering == PLB_ORDER_PENCIL) scfg.brick[0] = smp[s].nx;
+            uint32_t* inv = nullptr;
+            int32_t* t2 = nullptr;
+            rc = compute_order(scfg, smp[s].nx, smp[s].ny, smp[s].nz, ns, xyzs[s], c->stream, &inv,
+                               nullptr, nullptr);
+            if (!rc && hipMalloc(&t2, sizeof(int32_t) * (Q - 1) * ns) != hipSuccess) rc = PLB_ERR_NOMEM;
+            if (!rc) rc = permute_ref_table(tabs[s], ns, inv, t2, c->stream);
+            if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PLB_ERR_HIP;
+            if (inv) (void)hipFree(inv);
+            if (!rc) {
+                (void)hipFree(tabs[s]);
+                tabs[s] = t2;
+            } else if (t2) {
+                (void)hipFree(t2);
+            }
+        }
         b->off[s] = total;
         b->n[s] = ns;
         total += std::max<int64_t>(bs, (ns + bs - 1) / bs * bs);
@@ -104,7 +125,6 @@ extern "C" int plb_batch_create(plb_batch** out, const plb_config* cfg, const pl
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (n_samples < 1 || !samples) return PLB_ERR_INVALID;
-    if (cfg->ordering != PLB_ORDER_REFERENCE) return PLB_ERR_INVALID;
     for (int s = 0; s < n_samples; s++) {
         const plb_sample& p = samples[s];
         if (!p.flag || p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || p.nx > 65535 || p.ny > 65535 ||

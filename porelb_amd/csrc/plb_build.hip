@@ -316,10 +316,12 @@ __global__ void k_table_from_V(VArgs a)
 // ---------------------------------------------------------------- ordering + layout
 struct BrickArgs {
     int nx, ny, nz, bx, by, bz, nbx, nby, nbz;
+    int zfast;  // PENCIL: bricks visited z fastest, then y, then x
 };
 __device__ __forceinline__ int64_t brick_lin(const BrickArgs& b, int x, int y, int z)
 {
-    const int64_t brick = ((int64_t)(z / b.bz) * b.nby + (y / b.by)) * b.nbx + (x / b.bx);
+    const int64_t brick = b.zfast ? ((int64_t)(x / b.bx) * b.nby + (y / b.by)) * b.nbz + (z / b.bz)
+                                  : ((int64_t)(z / b.bz) * b.nby + (y / b.by)) * b.nbx + (x / b.bx);
     const int64_t inner = ((int64_t)(z % b.bz) * b.by + (y % b.by)) * b.bx + (x % b.bx);
     return brick * ((int64_t)b.bx * b.by * b.bz) + inner;
 }
@@ -374,6 +376,76 @@ __global__ void k_write_table(const int32_t* __restrict__ tabref, int64_t N,
 
 static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Node order of a lattice: inv_perm[g] = internal id of V-order node g,
+// perm = its inverse, xyz_int = coordinates in internal order.  For the
+// reference order nothing is allocated (identity).
+int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, const uint64_t* xyz_ref,
+                  hipStream_t s, uint32_t** inv_perm, uint32_t** perm, uint64_t* xyz_int)
+{
+    *inv_perm = nullptr;
+    if (perm) *perm = nullptr;
+    if (cfg.ordering == PLB_ORDER_REFERENCE || N == 0) {
+        if (xyz_int && N > 0)
+            PLB_HIP(hipMemcpyAsync(xyz_int, xyz_ref, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, s));
+        return PLB_OK;
+    }
+    BrickArgs b;
+    b.zfast = cfg.ordering == PLB_ORDER_PENCIL ? 1 : 0;
+    b.nx = nx;
+    b.ny = ny;
+    b.nz = nz;
+    // a brick edge longer than the domain only pads the bitmask
+    b.bx = std::min(cfg.brick[0], nx);
+    b.by = std::min(cfg.brick[1], ny);
+    b.bz = std::min(cfg.brick[2], nz);
+    b.nbx = (nx + b.bx - 1) / b.bx;
+    b.nby = (ny + b.by - 1) / b.by;
+    b.nbz = (nz + b.bz - 1) / b.bz;
+    const int64_t nvb = (int64_t)b.nbx * b.nby * b.nbz * b.bx * b.by * b.bz;
+    const int64_t nw = (nvb + 63) / 64;
+    uint64_t *bits = nullptr, *cnt = nullptr, *wpre = nullptr;
+    PLB_HIP(hipMalloc(&bits, sizeof(uint64_t) * nw));
+    PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * nw));
+    PLB_HIP(hipMalloc(&wpre, sizeof(uint64_t) * nw));
+    PLB_HIP(hipMemsetAsync(bits, 0, sizeof(uint64_t) * nw, s));
+    hipLaunchKernelGGL(k_brick_bits, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b,
+                       (unsigned long long*)bits);
+    hipLaunchKernelGGL(k_popc_words, dim3(nblk(nw, 256)), dim3(256), 0, s, bits, cnt, nw);
+    PLB_HIP(hipGetLastError());
+    int rc = exclusive_scan_u64(cnt, wpre, nw, s);
+    if (rc) return rc;
+    uint32_t* p = nullptr;
+    PLB_HIP(hipMalloc(inv_perm, sizeof(uint32_t) * N));
+    PLB_HIP(hipMalloc(&p, sizeof(uint32_t) * N));
+    uint64_t* xyz_tmp = nullptr;
+    if (!xyz_int) PLB_HIP(hipMalloc(&xyz_tmp, sizeof(uint64_t) * N));
+    hipLaunchKernelGGL(k_brick_ids, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b, bits, wpre,
+                       *inv_perm, p, xyz_int ? xyz_int : xyz_tmp);
+    PLB_HIP(hipGetLastError());
+    PLB_HIP(hipStreamSynchronize(s));
+    (void)hipFree(bits);
+    (void)hipFree(cnt);
+    (void)hipFree(wpre);
+    if (xyz_tmp) (void)hipFree(xyz_tmp);
+    if (perm)
+        *perm = p;
+    else
+        (void)hipFree(p);
+    return PLB_OK;
+}
+
+// V-order table [18][n] -> the same table in internal order with internal
+// ids, still [18][n] SoA (used for batch samples).
+int permute_ref_table(const int32_t* tabref, int64_t n, const uint32_t* inv_perm, int32_t* out,
+                      hipStream_t s)
+{
+    if (n == 0) return PLB_OK;
+    hipLaunchKernelGGL(k_write_table<PLB_LAYOUT_SOA>, dim3(nblk(n, 256)), dim3(256), 0, s, tabref, n,
+                       inv_perm, out, n);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
 int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref)
 {
     const int64_t N = c->N;
@@ -382,43 +454,9 @@ int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref)
     PLB_HIP(hipMalloc(&c->tab, sizeof(int32_t) * tab_elems));
     PLB_HIP(hipMemsetAsync(c->tab, 0, sizeof(int32_t) * tab_elems, s));
     PLB_HIP(hipMalloc(&c->node_xyz, sizeof(uint64_t) * (N ? N : 1)));
-    if (c->cfg.ordering == PLB_ORDER_BRICK && N > 0) {
-        BrickArgs b;
-        b.nx = c->nx;
-        b.ny = c->ny;
-        b.nz = c->nz;
-        b.bx = c->cfg.brick[0];
-        b.by = c->cfg.brick[1];
-        b.bz = c->cfg.brick[2];
-        b.nbx = (c->nx + b.bx - 1) / b.bx;
-        b.nby = (c->ny + b.by - 1) / b.by;
-        b.nbz = (c->nz + b.bz - 1) / b.bz;
-        const int64_t nvb = (int64_t)b.nbx * b.nby * b.nbz * b.bx * b.by * b.bz;
-        const int64_t nw = (nvb + 63) / 64;
-        uint64_t *bits = nullptr, *cnt = nullptr, *wpre = nullptr;
-        PLB_HIP(hipMalloc(&bits, sizeof(uint64_t) * nw));
-        PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * nw));
-        PLB_HIP(hipMalloc(&wpre, sizeof(uint64_t) * nw));
-        PLB_HIP(hipMemsetAsync(bits, 0, sizeof(uint64_t) * nw, s));
-        hipLaunchKernelGGL(k_brick_bits, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b,
-                           (unsigned long long*)bits);
-        hipLaunchKernelGGL(k_popc_words, dim3(nblk(nw, 256)), dim3(256), 0, s, bits, cnt, nw);
-        PLB_HIP(hipGetLastError());
-        int rc = exclusive_scan_u64(cnt, wpre, nw, s);
-        if (rc) return rc;
-        PLB_HIP(hipMalloc(&c->perm, sizeof(uint32_t) * N));
-        PLB_HIP(hipMalloc(&c->inv_perm, sizeof(uint32_t) * N));
-        hipLaunchKernelGGL(k_brick_ids, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b, bits,
-                           wpre, c->inv_perm, c->perm, c->node_xyz);
-        PLB_HIP(hipGetLastError());
-        PLB_HIP(hipStreamSynchronize(s));
-        PLB_HIP(hipFree(bits));
-        PLB_HIP(hipFree(cnt));
-        PLB_HIP(hipFree(wpre));
-    } else if (N > 0) {
-        PLB_HIP(hipMemcpyAsync(c->node_xyz, xyz_ref, sizeof(uint64_t) * N,
-                               hipMemcpyDeviceToDevice, s));
-    }
+    int rc = compute_order(c->cfg, c->nx, c->ny, c->nz, N, xyz_ref, s, &c->inv_perm, &c->perm,
+                           c->node_xyz);
+    if (rc) return rc;
     if (N > 0) {
         if (c->cfg.layout == PLB_LAYOUT_SOA)
             hipLaunchKernelGGL(k_write_table<PLB_LAYOUT_SOA>, dim3(nblk(N, 256)), dim3(256), 0, s,

@@ -117,6 +117,7 @@ struct plb_ctx {
 
 void plb_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 
+
 // Internal entry points shared between translation units.
 namespace plb {
 int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n);
@@ -141,6 +142,10 @@ int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int
 int check_cfg(const plb_config* cfg);
 int fp16_scale_log2_for(double gx);
 int compress_tile8(plb_ctx* c);
+int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, const uint64_t* xyz_ref,
+                  hipStream_t s, uint32_t** inv_perm, uint32_t** perm, uint64_t* xyz_int);
+int permute_ref_table(const int32_t* tabref, int64_t n, const uint32_t* inv_perm, int32_t* out,
+                      hipStream_t s);
 int ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, int num_p);
 int ctx_alloc_state(plb_ctx* c);
 }  // namespace plb

@@ -121,12 +121,15 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
         int layout, order, bx, by, bz, bs;
     };
     std::vector<Cand> cands;
-    const int bricks[][3] = {{0, 0, 0}, {16, 4, 4}, {32, 4, 2}, {8, 8, 8}, {64, 2, 2}};
+    // node orders: reference; x-fastest bricks; pencils of full rows
+    const int orders[][4] = {{PLB_ORDER_REFERENCE, 0, 0, 0}, {PLB_ORDER_BRICK, 16, 4, 4},
+                             {PLB_ORDER_BRICK, 8, 8, 8},      {PLB_ORDER_PENCIL, nx, 16, 1},
+                             {PLB_ORDER_PENCIL, nx, 64, 1},   {PLB_ORDER_PENCIL, nx, 128, 1}};
     for (int layout = 0; layout < 2; layout++)
-        for (const auto& b : bricks)
+        for (const auto& o : orders)
             for (int bs : {128, 256, 512})
-                cands.push_back({layout, b[0] ? 1 : 0, b[0] ? b[0] : base->brick[0],
-                                 b[0] ? b[1] : base->brick[1], b[0] ? b[2] : base->brick[2], bs});
+                cands.push_back({layout, o[0], o[0] ? o[1] : base->brick[0], o[0] ? o[2] : base->brick[1],
+                                 o[0] ? o[3] : base->brick[2], bs});
     double best_mlups = -1.0;
     int nres = 0;
     for (const Cand& cd : cands) {

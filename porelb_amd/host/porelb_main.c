@@ -16,7 +16,8 @@
  * environment; defaults are the reference's values:
  *   PLB_TAU (0.8)  PLB_GX (1e-3)  PLB_RHO0 (1)  PLB_SAVE (1000)
  *   PLB_PRECISION fp64|fp32|fp16 (fp64)  PLB_BB refcompat|halfway (refcompat)
- *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick  PLB_BLOCK (256)
+ *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick|pencil  PLB_BRICK bx,by,bz
+ *   PLB_INDEX i32|tile8  PLB_BLOCK (256)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
  *   PLB_CHECKPOINT file: populations written after the last step
  *   PLB_RESTART file: start from such a checkpoint (fills the empty
@@ -117,7 +118,15 @@ int main(int argc, char *argv[])
     cfg.precision = env_choice("PLB_PRECISION", "fp64", "fp32", "fp16", PLB_FP64);
     cfg.bb_mode = env_choice("PLB_BB", "refcompat", "halfway", NULL, PLB_BB_REFCOMPAT);
     cfg.layout = env_choice("PLB_LAYOUT", "soa", "aosoa", NULL, PLB_LAYOUT_SOA);
-    cfg.ordering = env_choice("PLB_ORDER", "reference", "brick", NULL, PLB_ORDER_REFERENCE);
+    cfg.ordering = env_choice("PLB_ORDER", "reference", "brick", "pencil", PLB_ORDER_REFERENCE);
+    {
+        const char *b = getenv("PLB_BRICK"); /* "bx,by,bz" */
+        if (b && *b && sscanf(b, "%d,%d,%d", &cfg.brick[0], &cfg.brick[1], &cfg.brick[2]) != 3) {
+            fprintf(stderr, ">>> bad value PLB_BRICK=%s\n", b);
+            exit(-10);
+        }
+    }
+    cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", NULL, PLB_INDEX_I32);
     cfg.block_size = env_int("PLB_BLOCK", cfg.block_size);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */

@@ -68,7 +68,7 @@ def test_table_from_V_and_flag_bitexact(plb, mode, np_):
     with plb.Lattice.from_flag(flag, np_, bb_mode=mode) as lat:
         assert np.array_equal(lat.read_table(), want)
     for layout, order in [(plb.LAYOUT_AOSOA, plb.ORDER_REFERENCE), (plb.LAYOUT_SOA, plb.ORDER_BRICK),
-                          (plb.LAYOUT_AOSOA, plb.ORDER_BRICK)]:
+                          (plb.LAYOUT_AOSOA, plb.ORDER_BRICK), (plb.LAYOUT_AOSOA, plb.ORDER_PENCIL)]:
         for index in (plb.INDEX_I32, plb.INDEX_TILE8):
             with plb.Lattice.from_flag(flag, np_, bb_mode=mode, layout=layout, ordering=order,
                                        brick=(4, 2, 3), index_kind=index) as lat:
@@ -139,7 +139,9 @@ def test_fp64_layouts_orders_blocks_identical(plb, mode):
     variants = [dict(), dict(layout=1), dict(ordering=1), dict(layout=1, ordering=1, brick=(8, 8, 8)),
                 dict(block_size=64), dict(block_size=1024), dict(ordering=1, brick=(40, 1, 1)),
                 dict(index_kind=1), dict(index_kind=1, layout=1, block_size=512),
-                dict(index_kind=1, layout=1, ordering=1, brick=(8, 8, 8))]
+                dict(index_kind=1, layout=1, ordering=1, brick=(8, 8, 8)),
+                dict(index_kind=1, layout=1, ordering=2, brick=(44, 4, 1)),
+                dict(ordering=2, brick=(1000, 7, 3))]
     for kw in variants:
         say("variant", mode, kw)
         with plb.Lattice.from_flag(flag, 2, tau=0.7, gx=1e-4, bb_mode=mode, **kw) as lat:
