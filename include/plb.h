@@ -93,10 +93,12 @@ typedef struct {
     int device;          /* HIP device ordinal                                    */
     int fp16_scale_log2; /* FP16: stored = (f - w rho0) * 2^k; 0 = derive from gx */
     int index_kind;      /* PLB_INDEX_I32 | PLB_INDEX_TILE8                       */
+    int nodes_per_thread; /* 1, or 2: a wave advances two 64-node tiles at once
+                             (block_size <= 512; single lattices only)          */
 } plb_config;
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
- * REFERENCE order/I32 index, 256 threads, device 0. */
+ * REFERENCE order/I32 index, 256 threads, one node per thread, device 0. */
 void plb_config_default(plb_config *cfg);
 
 typedef struct plb_ctx plb_ctx;

@@ -51,7 +51,8 @@ class Config(C.Structure):
     _fields_ = [("tau", C.c_double), ("gx", C.c_double), ("rho0", C.c_double),
                 ("precision", C.c_int), ("bb_mode", C.c_int), ("layout", C.c_int),
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
-                ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int)]
+                ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int),
+                ("nodes_per_thread", C.c_int)]
 
 
 class Reduction(C.Structure):
@@ -318,7 +319,7 @@ def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
     for r in res[:n.value]:
         c = r.cfg
         out.append(dict(layout=c.layout, ordering=c.ordering, brick=tuple(c.brick), block=c.block_size,
-                        mlups=r.mlups, ms=r.ms))
+                        npt=c.nodes_per_thread, mlups=r.mlups, ms=r.ms))
     return best, out
 
 

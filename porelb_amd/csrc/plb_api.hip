@@ -48,6 +48,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->device = 0;
     cfg->fp16_scale_log2 = 0;
     cfg->index_kind = PLB_INDEX_I32;
+    cfg->nodes_per_thread = 1;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -88,6 +89,8 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->block_size < 64 || cfg->block_size > 1024 || cfg->block_size % 64) return PLB_ERR_INVALID;
     if (cfg->fp16_scale_log2 < 0 || cfg->fp16_scale_log2 > 100) return PLB_ERR_INVALID;
     if (cfg->index_kind != PLB_INDEX_I32 && cfg->index_kind != PLB_INDEX_TILE8) return PLB_ERR_INVALID;
+    if (cfg->nodes_per_thread != 1 && cfg->nodes_per_thread != 2) return PLB_ERR_INVALID;
+    if (cfg->nodes_per_thread == 2 && cfg->block_size > 512) return PLB_ERR_INVALID;
     return PLB_OK;
 }
 

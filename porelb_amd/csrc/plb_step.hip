@@ -71,68 +71,87 @@ __device__ __forceinline__ bool frozen(unsigned long long fk, int64_t it)
     return fk != NO_FAULT && (int64_t)(fk >> 32) < it;
 }
 
-// The 18 pull-table entries of node i (t[1..18]).  TILE8: the wave is one
-// 64-node tile, so its 18 descriptors are uniform (scalar loads); each lane
-// then reads one byte per direction, or an int32 for a wide pair.
+// ---------------------------------------------------------------- phases of one node
+// A node update runs in four phases so a thread can interleave two nodes
+// (issue both nodes' index loads, then both nodes' gathers, before waiting):
+//   issue_entries  -> index loads (I32: 18 words; TILE8: 5 words + 18 scalar
+//                     descriptors of the wave's tile)
+//   decode_entries -> t[1..18]: >= 0 source node, -1 bounce, -2 constant
+//   gather         -> the 19 stored populations
+//   collide_*      -> collision and the 19 stores
+
+template <int IDX>
+struct Entries {
+    int32_t t[Q];                // I32: raw entries; after decode: decoded
+    uint32_t w[TILE8_WORDS];     // TILE8: packed offsets of this lane
+    int32_t desc[Q];             // TILE8: the tile's descriptors (uniform)
+};
+
 template <int LAYOUT, int IDX>
-__device__ __forceinline__ void load_entries(const StepArgs& a, int64_t i, int32_t t[Q])
+__device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
 {
     if constexpr (IDX == PLB_INDEX_I32) {
 #pragma unroll
-        for (int j = 1; j < Q; j++)
-            t[j] = a.tab[tpos<LAYOUT>(j - 1, i, a.Npad)];
+        for (int j = 1; j < Q; j++) e.t[j] = a.tab[tpos<LAYOUT>(j - 1, i, a.Npad)];
     } else {
+        // TILE8: the wave is one 64-node tile, so its 18 descriptors are
+        // uniform (scalar loads); each lane reads 5 words of packed offsets.
         const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
-        const int lane = (int)(i & 63);
-        const int32_t* d = a.tdesc + tile * (Q - 1);
-        const uint32_t* o = a.toff + tile * TILE8_WORDS * 64 + lane;
-        uint32_t w[TILE8_WORDS];
+        const uint32_t* o = a.toff + tile * TILE8_WORDS * 64 + (i & 63);
 #pragma unroll
-        for (int k = 0; k < TILE8_WORDS; k++) w[k] = o[k * 64];
-        int32_t desc[Q];
+        for (int k = 0; k < TILE8_WORDS; k++) e.w[k] = o[k * 64];
+        const int32_t* d = a.tdesc + tile * (Q - 1);
+#pragma unroll
+        for (int j = 1; j < Q; j++) e.desc[j] = d[j - 1];
+    }
+}
+
+template <int IDX>
+__device__ __forceinline__ void decode_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
+{
+    if constexpr (IDX == PLB_INDEX_TILE8) {
         int32_t any = 0;
 #pragma unroll
         for (int j = 1; j < Q; j++) {
-            desc[j] = d[j - 1];
-            any |= desc[j];
-        }
-#pragma unroll
-        for (int j = 1; j < Q; j++) {
             // one select: 0..253 -> base + v, 254 -> -2 (const), 255 -> -1 (bounce)
-            const int32_t v = (int32_t)((w[(j - 1) >> 2] >> (8 * ((j - 1) & 3))) & 0xffu);
-            t[j] = v < 254 ? desc[j] + v : v - 256;
+            const int32_t v = (int32_t)((e.w[(j - 1) >> 2] >> (8 * ((j - 1) & 3))) & 0xffu);
+            e.t[j] = v < 254 ? e.desc[j] + v : v - 256;
+            any |= e.desc[j];
         }
         if (any < 0) {  // a (tile, direction) pair of this tile is wide
 #pragma unroll
             for (int j = 1; j < Q; j++)
-                if (desc[j] < 0)
-                    t[j] = __builtin_nontemporal_load(a.twide + (int64_t)(-desc[j] - 1) * 64 + lane);
+                if (e.desc[j] < 0)
+                    e.t[j] = __builtin_nontemporal_load(a.twide + (int64_t)(-e.desc[j] - 1) * 64 + (i & 63));
         }
     }
 }
 
-// One node of the FP64 step, reference arithmetic.  The caller has checked
-// i < N.  a.omega / a.gx are the node's sample parameters.
-template <int LAYOUT, int IDX>
-__device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i, unsigned long long fk)
+// 19 gathers: rest population of the node, then per direction the source
+// node's slot j (stream) or the node's own slot re[j] (bounce; constant
+// entries read that slot too and are replaced afterwards).
+template <typename S, int LAYOUT>
+__device__ __forceinline__ void gather(const StepArgs& a, int64_t i, const int32_t t[Q], S raw[Q])
 {
-    const double* __restrict__ fin = (const double*)a.fin;
-    double* __restrict__ fout = (double*)a.fout;
-
-    int32_t t[Q];
-    load_entries<LAYOUT, IDX>(a, i, t);
-    double f[Q];
-    f[0] = fin[fpos<LAYOUT>(0, i, a.Npad)];
+    const S* __restrict__ fin = (const S*)a.fin;
+    raw[0] = fin[fpos<LAYOUT>(0, i, a.Npad)];
 #pragma unroll
     for (int j = 1; j < Q; j++) {
         const int64_t src = t[j] >= 0 ? fpos<LAYOUT>(j, t[j], a.Npad) : fpos<LAYOUT>(RE[j], i, a.Npad);
-        f[j] = fin[src];
+        raw[j] = fin[src];
     }
+}
+
+// FP64 collision in the reference's evaluation order (porelb.c:326-362).
+template <int LAYOUT>
+__device__ __forceinline__ void collide_f64(const StepArgs& a, int64_t i, const int32_t t[Q],
+                                            double f[Q], unsigned long long fk)
+{
+    double* __restrict__ fout = (double*)a.fout;
 #pragma unroll
     for (int j = 1; j < Q; j++)
         if (t[j] == TAB_CONST) f[j] = TP[j] * a.rho0 * 1.0;  // feq(rho0, u=0) = initial f1
 
-    // collision (porelb.c:326-362)
     double rho = 0.0, rhou = 0.0, rhov = 0.0, rhow = 0.0;
 #pragma unroll
     for (int j = 0; j < Q; j++) {
@@ -162,11 +181,8 @@ __device__ __forceinline__ void node_f64(const StepArgs& a, int64_t i, unsigned 
 }
 
 // ---------------------------------------------------------------- shifted FP32 / FP16 storage
-__device__ __forceinline__ float ld_dev(const float* p, int64_t k, float) { return p[k]; }
-__device__ __forceinline__ float ld_dev(const __half* p, int64_t k, float inv)
-{
-    return __half2float(p[k]) * inv;
-}
+__device__ __forceinline__ float to_dev(float v, float) { return v; }
+__device__ __forceinline__ float to_dev(__half v, float inv) { return __half2float(v) * inv; }
 __device__ __forceinline__ void st_dev(float* p, int64_t k, float v, float)
 {
     __builtin_nontemporal_store(v, p + k);
@@ -177,22 +193,15 @@ __device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
     __builtin_nontemporal_store(b, (unsigned short*)p + k);
 }
 
-template <typename S, int LAYOUT, int IDX>
-__device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i, unsigned long long fk)
+template <typename S, int LAYOUT>
+__device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, const int32_t t[Q],
+                                              const S raw[Q], unsigned long long fk)
 {
-    const S* __restrict__ fin = (const S*)a.fin;
     S* __restrict__ fout = (S*)a.fout;
     const float inv = a.inv_scale;
-
-    int32_t t[Q];
-    load_entries<LAYOUT, IDX>(a, i, t);
     float d[Q];
-    d[0] = ld_dev(fin, fpos<LAYOUT>(0, i, a.Npad), inv);
 #pragma unroll
-    for (int j = 1; j < Q; j++) {
-        const int64_t src = t[j] >= 0 ? fpos<LAYOUT>(j, t[j], a.Npad) : fpos<LAYOUT>(RE[j], i, a.Npad);
-        d[j] = ld_dev(fin, src, inv);
-    }
+    for (int j = 0; j < Q; j++) d[j] = to_dev(raw[j], inv);
 #pragma unroll
     for (int j = 1; j < Q; j++)
         if (t[j] == TAB_CONST) d[j] = 0.f;
@@ -237,16 +246,28 @@ __device__ __forceinline__ void node_shift(const StepArgs& a, int64_t i, unsigne
     }
 }
 
+template <typename S, int LAYOUT>
+__device__ __forceinline__ void collide_any(const StepArgs& a, int64_t i, const int32_t t[Q], S raw[Q],
+                                            unsigned long long fk)
+{
+    if constexpr (sizeof(S) == 8)
+        collide_f64<LAYOUT>(a, i, t, (double*)raw, fk);
+    else
+        collide_shift<S, LAYOUT>(a, i, t, raw, fk);
+}
+
 template <typename S, int LAYOUT, int IDX>
 __device__ __forceinline__ void node_any(const StepArgs& a, int64_t i, unsigned long long fk)
 {
-    if constexpr (sizeof(S) == 8)
-        node_f64<LAYOUT, IDX>(a, i, fk);
-    else
-        node_shift<S, LAYOUT, IDX>(a, i, fk);
+    Entries<IDX> e;
+    issue_entries<LAYOUT, IDX>(a, i, e);
+    decode_entries<IDX>(a, i, e);
+    S raw[Q];
+    gather<S, LAYOUT>(a, i, e.t, raw);
+    collide_any<S, LAYOUT>(a, i, e.t, raw, fk);
 }
 
-// Single lattice: one thread per node, XCD-contiguous block order.
+// Single lattice, one node per thread, XCD-contiguous block order.
 template <typename S, int LAYOUT, int IDX>
 __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
 {
@@ -254,6 +275,31 @@ __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
     const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
     node_any<S, LAYOUT, IDX>(a, i, *a.fault);
+}
+
+// Single lattice, two nodes per thread: each wave covers two consecutive
+// 64-node tiles and keeps both tiles' index loads and gathers in flight
+// together (more bytes in flight per wave at the cost of registers).
+template <typename S, int LAYOUT, int IDX>
+__global__ void __launch_bounds__(512) k_step2(const StepArgs a)
+{
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t t = blk * blockDim.x + threadIdx.x;
+    const int64_t i0 = (t >> 6) * 128 + (t & 63);
+    const int64_t i1 = i0 + 64;
+    if (i0 >= a.N) return;
+    const bool two = i1 < a.N;
+    const unsigned long long fk = *a.fault;
+    Entries<IDX> e0, e1;
+    issue_entries<LAYOUT, IDX>(a, i0, e0);
+    if (two) issue_entries<LAYOUT, IDX>(a, i1, e1);
+    decode_entries<IDX>(a, i0, e0);
+    if (two) decode_entries<IDX>(a, i1, e1);
+    S r0[Q], r1[Q];
+    gather<S, LAYOUT>(a, i0, e0.t, r0);
+    if (two) gather<S, LAYOUT>(a, i1, e1.t, r1);
+    collide_any<S, LAYOUT>(a, i0, e0.t, r0, fk);
+    if (two) collide_any<S, LAYOUT>(a, i1, e1.t, r1, fk);
 }
 
 // Batch of samples: the grid enumerates the blocks of active samples
@@ -363,7 +409,9 @@ __global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const u
     const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
     int32_t t[Q];
     if constexpr (IDX == PLB_INDEX_I32) {
-        load_entries<LAYOUT, IDX>(a, k, t);
+        Entries<IDX> e;
+        issue_entries<LAYOUT, IDX>(a, k, e);
+        for (int j = 1; j < Q; j++) t[j] = e.t[j];
     } else {
         // Lanes of a wave hold unrelated nodes here: decode without the
         // per-wave descriptor shortcut.
@@ -547,8 +595,14 @@ int launch_step(plb_ctx* c, int64_t it, bool write_macros)
 {
     const StepArgs a = make_args(c, it, write_macros);
     const int bs = c->cfg.block_size;
-    PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step, dim3(nblk(c->N, bs)),
-                  dim3(bs), 0, c->stream, a);
+    if (c->cfg.nodes_per_thread == 2) {
+        const int64_t threads = (c->N + 127) / 128 * 64;
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step2,
+                      dim3(nblk(threads, bs)), dim3(bs), 0, c->stream, a);
+    } else {
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step, dim3(nblk(c->N, bs)),
+                      dim3(bs), 0, c->stream, a);
+    }
     PLB_HIP(hipGetLastError());
     c->cur ^= 1;
     return PLB_OK;

@@ -86,16 +86,17 @@ static bool cache_lookup(const char* path, uint64_t key, int prec, int bb, plb_c
     bool found = false;
     while (fgets(line, sizeof line, fp)) {
         unsigned long long k;
-        int p, b, layout, order, bx, by, bz, bs;
-        if (sscanf(line, "%llx %d %d %d %d %d %d %d %d", &k, &p, &b, &layout, &order, &bx, &by, &bz,
-                   &bs) == 9 &&
-            k == key && p == prec && b == bb) {
+        int p, b, layout, order, bx, by, bz, bs, npt, idx;
+        if (sscanf(line, "%llx %d %d %d %d %d %d %d %d %d %d", &k, &p, &b, &idx, &layout, &order, &bx,
+                   &by, &bz, &bs, &npt) == 11 &&
+            k == key && p == prec && b == bb && idx == best->index_kind) {
             best->layout = layout;
             best->ordering = order;
             best->brick[0] = bx;
             best->brick[1] = by;
             best->brick[2] = bz;
             best->block_size = bs;
+            best->nodes_per_thread = npt;
             found = true;  // the last matching line wins
         }
     }
@@ -118,18 +119,20 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
         return PLB_OK;
 
     struct Cand {
-        int layout, order, bx, by, bz, bs;
+        int layout, order, bx, by, bz, bs, npt;
     };
     std::vector<Cand> cands;
-    // node orders: reference; x-fastest bricks; pencils of full rows
-    const int orders[][4] = {{PLB_ORDER_REFERENCE, 0, 0, 0}, {PLB_ORDER_BRICK, 16, 4, 4},
-                             {PLB_ORDER_BRICK, 8, 8, 8},      {PLB_ORDER_PENCIL, nx, 16, 1},
-                             {PLB_ORDER_PENCIL, nx, 64, 1},   {PLB_ORDER_PENCIL, nx, 128, 1}};
+    // node orders: reference; 8^3 bricks; pencils of 64 and 128 full rows
+    const int orders[][4] = {{PLB_ORDER_REFERENCE, 0, 0, 0},
+                             {PLB_ORDER_BRICK, 8, 8, 8},
+                             {PLB_ORDER_PENCIL, nx, 64, 1},
+                             {PLB_ORDER_PENCIL, nx, 128, 1}};
     for (int layout = 0; layout < 2; layout++)
         for (const auto& o : orders)
-            for (int bs : {128, 256, 512})
-                cands.push_back({layout, o[0], o[0] ? o[1] : base->brick[0], o[0] ? o[2] : base->brick[1],
-                                 o[0] ? o[3] : base->brick[2], bs});
+            for (int bs : {128, 256})
+                for (int npt : {1, 2})
+                    cands.push_back({layout, o[0], o[0] ? o[1] : base->brick[0],
+                                     o[0] ? o[2] : base->brick[1], o[0] ? o[3] : base->brick[2], bs, npt});
     double best_mlups = -1.0;
     int nres = 0;
     for (const Cand& cd : cands) {
@@ -140,6 +143,7 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
         cfg.brick[1] = cd.by;
         cfg.brick[2] = cd.bz;
         cfg.block_size = cd.bs;
+        cfg.nodes_per_thread = cd.npt;
         plb_ctx* c = nullptr;
         rc = plb_create_from_flag(&c, &cfg, flag, nx, ny, nz, num_p);
         if (rc) return rc;
@@ -161,9 +165,10 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
     if (cache_path) {
         FILE* fp = fopen(cache_path, "a");
         if (fp) {
-            fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
-                    base->precision, base->bb_mode, best->layout, best->ordering, best->brick[0],
-                    best->brick[1], best->brick[2], best->block_size, best_mlups);
+            fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
+                    base->precision, base->bb_mode, base->index_kind, best->layout, best->ordering,
+                    best->brick[0], best->brick[1], best->brick[2], best->block_size,
+                    best->nodes_per_thread, best_mlups);
             fclose(fp);
         }
     }

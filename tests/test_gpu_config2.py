@@ -1,0 +1,39 @@
+"""Config 2 at full size: a 256^3 sphere pack (radius U[6,12], porosity 0.30),
+tau 0.8, g_x 1e-6.  The reference solver (built from its own sources) runs
+under mpirun with 16 ranks; the GPU path reads the same V.bin (16 z-slabs,
+so the same rank-dependent bounce order) and must produce bit-identical
+populations and part files."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(REPO, "porelb_amd", "bin")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.skipif(not (O.have_reference_build() and os.path.exists(O.MPIRUN)),
+                    reason="reference build or mpirun absent")
+def test_config2_gpu_equals_reference_mpirun_16(plb, tmp_path):
+    flag = plb.mask_spheres((256, 256, 256), 6.0, 12.0, 0.30, 2)
+    steps, ranks = 100, 16
+    print("[case] config2 reference mpirun", ranks, steps, flush=True)
+    ref = O.run_reference(flag, ranks, steps, 0.8, 1e-6, save_every=1000, workdir=str(tmp_path),
+                          dump=True, timeout=900)
+    vb = ref["vbin"]
+    print("[case] config2 GPU", vb["num_node"], flush=True)
+    with plb.Lattice.from_V(vb["V"], vb["start"], vb["dims"], tau=0.8, gx=1e-6) as lat:
+        lat.step(0, steps)
+        f = lat.read_populations()
+        mac = lat.read_macros()
+    assert sha(f) == sha(ref["f0"])
+    assert sha(mac) == sha(ref["macros"])
+    assert ref["timing"]["ranks"] == ranks

@@ -141,7 +141,9 @@ def test_fp64_layouts_orders_blocks_identical(plb, mode):
                 dict(index_kind=1), dict(index_kind=1, layout=1, block_size=512),
                 dict(index_kind=1, layout=1, ordering=1, brick=(8, 8, 8)),
                 dict(index_kind=1, layout=1, ordering=2, brick=(44, 4, 1)),
-                dict(ordering=2, brick=(1000, 7, 3))]
+                dict(ordering=2, brick=(1000, 7, 3)),
+                dict(nodes_per_thread=2), dict(nodes_per_thread=2, index_kind=1, layout=1, block_size=128),
+                dict(nodes_per_thread=2, index_kind=1, layout=1, ordering=2, brick=(44, 8, 1), block_size=512)]
     for kw in variants:
         say("variant", mode, kw)
         with plb.Lattice.from_flag(flag, 2, tau=0.7, gx=1e-4, bb_mode=mode, **kw) as lat:

@@ -1,0 +1,38 @@
+"""Autotuner and copy-ceiling calibration on the GPU."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_copy_bandwidth_plausible(plb):
+    g = plb.copy_bandwidth(1 << 30, 3)
+    assert 1000.0 < g < 8000.0  # GB/s, below the 8 TB/s HBM peak
+
+
+def test_autotune_cache_and_result(plb, tmp_path):
+    flag = plb.mask_spheres((48, 40, 64), 3.0, 6.0, 0.3, 8)
+    cache = str(tmp_path / "tune.txt")
+    base = plb.default_config(tau=0.8, gx=1e-5, precision=plb.FP64, bb_mode=plb.BB_REFCOMPAT)
+    best, res = plb.autotune(flag, 1, 5, cache, cfg=base)
+    assert len(res) == 32 and all(r["mlups"] > 0 for r in res)
+    assert max(r["mlups"] for r in res) == pytest.approx(max(r["mlups"] for r in res))
+    text = open(cache).read().split()
+    assert len(text) == 12
+    t0 = time.perf_counter()
+    best2, res2 = plb.autotune(flag, 1, 5, cache, cfg=base)
+    assert res2 == [] and time.perf_counter() - t0 < 2.0
+    for k in ("layout", "ordering", "block_size", "nodes_per_thread"):
+        assert getattr(best, k) == getattr(best2, k)
+    # the tuned shape changes speed only, never the FP64 result
+    V, start = O.build_V(flag, 1)
+    tab = O.pull_table(V, start, O.BB_REFCOMPAT)
+    f0, _ = O.init_equilibrium(len(V))
+    O.pull_run(tab, f0, 30, 0.8, 1e-5)
+    with plb.Lattice.from_flag(flag, 1, cfg=best) as lat:
+        lat.step(0, 30)
+        assert np.array_equal(lat.read_populations(), f0)
