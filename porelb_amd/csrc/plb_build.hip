@@ -83,19 +83,17 @@ int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t
         PLB_HIP(hipGetLastError());
         return PLB_OK;
     }
-    uint64_t *bsum = nullptr, *boff = nullptr;
-    PLB_HIP(hipMalloc(&bsum, sizeof(uint64_t) * nb));
-    PLB_HIP(hipMalloc(&boff, sizeof(uint64_t) * nb));
-    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, bsum, n);
+    DevBuf<uint64_t> bsum, boff;
+    PLB_HIP(bsum.alloc(nb));
+    PLB_HIP(boff.alloc(nb));
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, bsum.p, n);
     PLB_HIP(hipGetLastError());
     int rc = exclusive_scan_u64(bsum, boff, nb, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, boff,
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, boff.p,
                        n);
     PLB_HIP(hipGetLastError());
     PLB_HIP(hipStreamSynchronize(s));
-    PLB_HIP(hipFree(bsum));
-    PLB_HIP(hipFree(boff));
     return PLB_OK;
 }
 
@@ -403,34 +401,27 @@ int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, cons
     b.nbz = (nz + b.bz - 1) / b.bz;
     const int64_t nvb = (int64_t)b.nbx * b.nby * b.nbz * b.bx * b.by * b.bz;
     const int64_t nw = (nvb + 63) / 64;
-    uint64_t *bits = nullptr, *cnt = nullptr, *wpre = nullptr;
-    PLB_HIP(hipMalloc(&bits, sizeof(uint64_t) * nw));
-    PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * nw));
-    PLB_HIP(hipMalloc(&wpre, sizeof(uint64_t) * nw));
+    DevBuf<uint64_t> bits, cnt, wpre, xyz_tmp;
+    DevBuf<uint32_t> inv, p;
+    PLB_HIP(bits.alloc(nw));
+    PLB_HIP(cnt.alloc(nw));
+    PLB_HIP(wpre.alloc(nw));
     PLB_HIP(hipMemsetAsync(bits, 0, sizeof(uint64_t) * nw, s));
     hipLaunchKernelGGL(k_brick_bits, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b,
-                       (unsigned long long*)bits);
-    hipLaunchKernelGGL(k_popc_words, dim3(nblk(nw, 256)), dim3(256), 0, s, bits, cnt, nw);
+                       (unsigned long long*)bits.p);
+    hipLaunchKernelGGL(k_popc_words, dim3(nblk(nw, 256)), dim3(256), 0, s, bits.p, cnt.p, nw);
     PLB_HIP(hipGetLastError());
     int rc = exclusive_scan_u64(cnt, wpre, nw, s);
     if (rc) return rc;
-    uint32_t* p = nullptr;
-    PLB_HIP(hipMalloc(inv_perm, sizeof(uint32_t) * N));
-    PLB_HIP(hipMalloc(&p, sizeof(uint32_t) * N));
-    uint64_t* xyz_tmp = nullptr;
-    if (!xyz_int) PLB_HIP(hipMalloc(&xyz_tmp, sizeof(uint64_t) * N));
-    hipLaunchKernelGGL(k_brick_ids, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b, bits, wpre,
-                       *inv_perm, p, xyz_int ? xyz_int : xyz_tmp);
+    PLB_HIP(inv.alloc(N));
+    PLB_HIP(p.alloc(N));
+    if (!xyz_int) PLB_HIP(xyz_tmp.alloc(N));
+    hipLaunchKernelGGL(k_brick_ids, dim3(nblk(N, 256)), dim3(256), 0, s, xyz_ref, N, b, bits.p, wpre.p,
+                       inv.p, p.p, xyz_int ? xyz_int : xyz_tmp.p);
     PLB_HIP(hipGetLastError());
     PLB_HIP(hipStreamSynchronize(s));
-    (void)hipFree(bits);
-    (void)hipFree(cnt);
-    (void)hipFree(wpre);
-    if (xyz_tmp) (void)hipFree(xyz_tmp);
-    if (perm)
-        *perm = p;
-    else
-        (void)hipFree(p);
+    *inv_perm = inv.release();
+    if (perm) *perm = p.release();
     return PLB_OK;
 }
 
@@ -555,21 +546,21 @@ int compress_tile8(plb_ctx* c)
     hipStream_t s = c->stream;
     const int64_t T = c->Npad / 64;
     const int64_t nk = T * (Q - 1);
-    uint64_t *wide = nullptr, *slot = nullptr;
-    PLB_HIP(hipMalloc(&wide, sizeof(uint64_t) * nk));
-    PLB_HIP(hipMalloc(&slot, sizeof(uint64_t) * nk));
+    DevBuf<uint64_t> wide, slot;
+    PLB_HIP(wide.alloc(nk));
+    PLB_HIP(slot.alloc(nk));
     if (c->cfg.layout == PLB_LAYOUT_SOA)
         hipLaunchKernelGGL(k_tile8_flags<PLB_LAYOUT_SOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide);
+                           c->Npad, c->N, wide.p);
     else
         hipLaunchKernelGGL(k_tile8_flags<PLB_LAYOUT_AOSOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide);
+                           c->Npad, c->N, wide.p);
     PLB_HIP(hipGetLastError());
     int rc = exclusive_scan_u64(wide, slot, nk, s);
     if (rc) return rc;
     uint64_t last_slot = 0, last_wide = 0;
-    PLB_HIP(hipMemcpyAsync(&last_slot, slot + nk - 1, 8, hipMemcpyDeviceToHost, s));
-    PLB_HIP(hipMemcpyAsync(&last_wide, wide + nk - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&last_slot, slot.p + nk - 1, 8, hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&last_wide, wide.p + nk - 1, 8, hipMemcpyDeviceToHost, s));
     PLB_HIP(hipStreamSynchronize(s));
     c->n_wide = (int64_t)(last_slot + last_wide);
     PLB_HIP(hipMalloc(&c->tdesc, sizeof(int32_t) * nk));
@@ -577,14 +568,12 @@ int compress_tile8(plb_ctx* c)
     PLB_HIP(hipMalloc(&c->twide, sizeof(int32_t) * 64 * (c->n_wide ? c->n_wide : 1)));
     if (c->cfg.layout == PLB_LAYOUT_SOA)
         hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_SOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide, slot, c->tdesc, c->toff, c->twide);
+                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->twide);
     else
         hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_AOSOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide, slot, c->tdesc, c->toff, c->twide);
+                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->twide);
     PLB_HIP(hipGetLastError());
     PLB_HIP(hipStreamSynchronize(s));
-    (void)hipFree(wide);
-    (void)hipFree(slot);
     (void)hipFree(c->tab);
     c->tab = nullptr;
     c->index_bytes = c->N ? ((double)T * TILE8_WORDS * 256 + (double)nk * 4 + (double)c->n_wide * 256) /
@@ -596,97 +585,82 @@ int compress_tile8(plb_ctx* c)
 int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n)
 {
     hipStream_t s = c->stream;
-    uint64_t* dV = nullptr;
-    int32_t *dstart = nullptr, *tabref = nullptr;
-    uint64_t* xyz = nullptr;
-    int* derr = nullptr;
-    const int64_t nn = n ? n : 1;
-    PLB_HIP(hipMalloc(&dV, sizeof(plb_node_info) * nn));
-    PLB_HIP(hipMalloc(&dstart, sizeof(int32_t) * c->num_p));
-    PLB_HIP(hipMalloc(&tabref, sizeof(int32_t) * (Q - 1) * nn));
-    PLB_HIP(hipMalloc(&xyz, sizeof(uint64_t) * nn));
-    PLB_HIP(hipMalloc(&derr, sizeof(int)));
-    PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
-    if (n) PLB_HIP(hipMemcpyAsync(dV, V, sizeof(plb_node_info) * n, hipMemcpyHostToDevice, s));
-    PLB_HIP(hipMemcpyAsync(dstart, c->V_sub_start.data(), sizeof(int32_t) * c->num_p,
-                           hipMemcpyHostToDevice, s));
-    if (n) {
-        VArgs a;
-        a.V = dV;
-        a.N = n;
-        a.start = dstart;
-        a.num_p = c->num_p;
-        a.nx = c->nx;
-        a.ny = c->ny;
-        a.nz = c->nz;
-        a.mode = c->cfg.bb_mode;
-        a.tab = tabref;
-        a.xyz = xyz;
-        a.err = derr;
-        hipLaunchKernelGGL(k_table_from_V, dim3(nblk(n, 256)), dim3(256), 0, s, a);
-        PLB_HIP(hipGetLastError());
-    }
+    DevBuf<uint64_t> xyz;
+    DevBuf<int32_t> dstart, tabref;
+    DevBuf<int> derr;
     int herr = 0;
-    PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
-    PLB_HIP(hipStreamSynchronize(s));
-    PLB_HIP(hipFree(dV));
-    int rc = herr ? herr : finalize_table(c, tabref, xyz);
-    (void)hipFree(tabref);
-    (void)hipFree(xyz);
-    (void)hipFree(dstart);
-    (void)hipFree(derr);
-    return rc;
+    {
+        DevBuf<uint64_t> dV;  // 152-byte records as 19 words; freed before the final table exists
+        PLB_HIP(dV.alloc(n * 19));
+        PLB_HIP(dstart.alloc(c->num_p));
+        PLB_HIP(tabref.alloc((Q - 1) * n));
+        PLB_HIP(xyz.alloc(n));
+        PLB_HIP(derr.alloc(1));
+        PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+        if (n) PLB_HIP(hipMemcpyAsync(dV, V, sizeof(plb_node_info) * n, hipMemcpyHostToDevice, s));
+        PLB_HIP(hipMemcpyAsync(dstart, c->V_sub_start.data(), sizeof(int32_t) * c->num_p,
+                               hipMemcpyHostToDevice, s));
+        if (n) {
+            VArgs a;
+            a.V = dV;
+            a.N = n;
+            a.start = dstart;
+            a.num_p = c->num_p;
+            a.nx = c->nx;
+            a.ny = c->ny;
+            a.nz = c->nz;
+            a.mode = c->cfg.bb_mode;
+            a.tab = tabref;
+            a.xyz = xyz;
+            a.err = derr;
+            hipLaunchKernelGGL(k_table_from_V, dim3(nblk(n, 256)), dim3(256), 0, s, a);
+            PLB_HIP(hipGetLastError());
+        }
+        PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+        PLB_HIP(hipStreamSynchronize(s));
+    }
+    return herr ? herr : finalize_table(c, tabref, xyz);
 }
 
 // Device-side mask analysis shared by plb_create_from_flag and plb_preprocess.
 struct FlagIndex {
-    uint64_t* bits = nullptr;
-    uint64_t* wpre = nullptr;
-    uint64_t* xyz = nullptr;
+    DevBuf<uint64_t> bits, wpre, xyz;
     int64_t N = 0, nvox = 0, nwords = 0;
-    void release()
-    {
-        if (bits) (void)hipFree(bits);
-        if (wpre) (void)hipFree(wpre);
-        if (xyz) (void)hipFree(xyz);
-        bits = wpre = xyz = nullptr;
-    }
 };
 
 static int index_flag(const uint8_t* flag, int nx, int ny, int nz, hipStream_t s, FlagIndex& fi)
 {
     fi.nvox = (int64_t)nx * ny * nz;
     fi.nwords = (fi.nvox + 63) / 64;
-    uint8_t* dflag = nullptr;
-    uint64_t* cnt = nullptr;
-    int* derr = nullptr;
-    PLB_HIP(hipMalloc(&dflag, fi.nvox));
-    PLB_HIP(hipMalloc(&fi.bits, sizeof(uint64_t) * fi.nwords));
-    PLB_HIP(hipMalloc(&cnt, sizeof(uint64_t) * fi.nwords));
-    PLB_HIP(hipMalloc(&fi.wpre, sizeof(uint64_t) * fi.nwords));
-    PLB_HIP(hipMalloc(&derr, sizeof(int)));
-    PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
-    PLB_HIP(hipMemcpyAsync(dflag, flag, fi.nvox, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_flag_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, dflag, fi.nvox,
-                       fi.nwords, fi.bits, cnt, derr);
-    PLB_HIP(hipGetLastError());
-    int rc = exclusive_scan_u64(cnt, fi.wpre, fi.nwords, s);
-    if (rc) return rc;
     uint64_t last_pre = 0, last_cnt = 0;
     int herr = 0;
-    PLB_HIP(hipMemcpyAsync(&last_pre, fi.wpre + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
-    PLB_HIP(hipMemcpyAsync(&last_cnt, cnt + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
-    PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
-    PLB_HIP(hipStreamSynchronize(s));
-    (void)hipFree(dflag);
-    (void)hipFree(cnt);
-    (void)hipFree(derr);
+    {
+        DevBuf<uint8_t> dflag;
+        DevBuf<uint64_t> cnt;
+        DevBuf<int> derr;
+        PLB_HIP(dflag.alloc(fi.nvox));
+        PLB_HIP(fi.bits.alloc(fi.nwords));
+        PLB_HIP(cnt.alloc(fi.nwords));
+        PLB_HIP(fi.wpre.alloc(fi.nwords));
+        PLB_HIP(derr.alloc(1));
+        PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+        PLB_HIP(hipMemcpyAsync(dflag, flag, fi.nvox, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_flag_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, dflag.p, fi.nvox,
+                           fi.nwords, fi.bits.p, cnt.p, derr.p);
+        PLB_HIP(hipGetLastError());
+        int rc = exclusive_scan_u64(cnt, fi.wpre, fi.nwords, s);
+        if (rc) return rc;
+        PLB_HIP(hipMemcpyAsync(&last_pre, fi.wpre.p + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
+        PLB_HIP(hipMemcpyAsync(&last_cnt, cnt.p + fi.nwords - 1, 8, hipMemcpyDeviceToHost, s));
+        PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+        PLB_HIP(hipStreamSynchronize(s));
+    }
     if (herr) return herr;
     fi.N = (int64_t)(last_pre + last_cnt);
     if (fi.N >= INT32_MAX) return PLB_ERR_RANGE;
-    PLB_HIP(hipMalloc(&fi.xyz, sizeof(uint64_t) * (fi.N ? fi.N : 1)));
-    hipLaunchKernelGGL(k_nodes_from_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, fi.bits,
-                       fi.wpre, fi.nvox, nx, ny, fi.xyz);
+    PLB_HIP(fi.xyz.alloc(fi.N));
+    hipLaunchKernelGGL(k_nodes_from_bits, dim3(nblk(fi.nvox, 256)), dim3(256), 0, s, fi.bits.p,
+                       fi.wpre.p, fi.nvox, nx, ny, fi.xyz.p);
     PLB_HIP(hipGetLastError());
     return PLB_OK;
 }
@@ -701,10 +675,7 @@ int build_ref_from_flag(const uint8_t* flag, int nx, int ny, int nz, int num_p, 
     *xyz_out = nullptr;
     FlagIndex fi;
     int rc = index_flag(flag, nx, ny, nz, s, fi);
-    if (rc) {
-        fi.release();
-        return rc;
-    }
+    if (rc) return rc;
     GridArgs a;
     a.bits = fi.bits;
     a.wpre = fi.wpre;
@@ -714,53 +685,34 @@ int build_ref_from_flag(const uint8_t* flag, int nx, int ny, int nz, int num_p, 
     a.ny = ny;
     a.nz = nz;
     a.slab = nz / num_p;
-    int32_t* dstart = nullptr;
-    int32_t* tabref = nullptr;
-    hipError_t e = hipMalloc(&dstart, sizeof(int32_t) * num_p);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p, dstart);
-        start_out->assign(num_p, 0);
-        e = hipMemcpyAsync(start_out->data(), dstart, sizeof(int32_t) * num_p,
-                           hipMemcpyDeviceToHost, s);
+    DevBuf<int32_t> dstart, tabref;
+    PLB_HIP(dstart.alloc(num_p));
+    hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p, dstart.p);
+    start_out->assign(num_p, 0);
+    PLB_HIP(hipMemcpyAsync(start_out->data(), dstart, sizeof(int32_t) * num_p, hipMemcpyDeviceToHost, s));
+    PLB_HIP(tabref.alloc((Q - 1) * fi.N));
+    if (fi.N) {
+        hipLaunchKernelGGL(k_table_from_bits, dim3(nblk(fi.N, 256)), dim3(256), 0, s, a, mode, tabref.p);
+        PLB_HIP(hipGetLastError());
     }
-    if (e == hipSuccess) e = hipMalloc(&tabref, sizeof(int32_t) * (Q - 1) * (fi.N ? fi.N : 1));
-    if (e == hipSuccess && fi.N) {
-        hipLaunchKernelGGL(k_table_from_bits, dim3(nblk(fi.N, 256)), dim3(256), 0, s, a, mode,
-                           tabref);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (dstart) (void)hipFree(dstart);
-    (void)hipFree(fi.bits);
-    (void)hipFree(fi.wpre);
-    fi.bits = fi.wpre = nullptr;
-    if (e != hipSuccess) {
-        plb_set_hip_error(e, "build_ref_from_flag", __FILE__, __LINE__);
-        if (tabref) (void)hipFree(tabref);
-        fi.release();
-        return e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
-    }
-    *tabref_out = tabref;
-    *xyz_out = fi.xyz;
-    fi.xyz = nullptr;
+    PLB_HIP(hipStreamSynchronize(s));
+    *tabref_out = tabref.release();
+    *xyz_out = fi.xyz.release();
     *n_out = fi.N;
     return PLB_OK;
 }
 
 int build_table_from_flag(plb_ctx* c, const uint8_t* flag)
 {
-    int32_t* tabref = nullptr;
-    uint64_t* xyz = nullptr;
+    DevBuf<int32_t> tabref;
+    DevBuf<uint64_t> xyz;
     int64_t n = 0;
     int rc = build_ref_from_flag(flag, c->nx, c->ny, c->nz, c->num_p, c->cfg.bb_mode, c->stream,
-                                 &tabref, &xyz, &n, &c->V_sub_start);
+                                 &tabref.p, &xyz.p, &n, &c->V_sub_start);
     if (rc) return rc;
     c->N = n;
     c->Npad = std::max<int64_t>(64, (n + 63) / 64 * 64);
-    rc = finalize_table(c, tabref, xyz);
-    (void)hipFree(tabref);
-    (void)hipFree(xyz);
-    return rc;
+    return finalize_table(c, tabref, xyz);
 }
 
 template <int LAYOUT>
@@ -796,6 +748,49 @@ int copy_table_offset(int layout, const int32_t* tabref, int64_t n, int64_t off,
 
 using namespace plb;
 
+// Device preprocess on its own stream; the stream is destroyed on every path.
+static int preprocess_on(hipStream_t s, const uint8_t* flag, int nx, int ny, int nz, int num_p,
+                         plb_node_info* V_out, int32_t* V_sub_start_out, int64_t* n_nodes_out)
+{
+    FlagIndex fi;
+    int rc = index_flag(flag, nx, ny, nz, s, fi);
+    if (rc) return rc;
+    if (n_nodes_out) *n_nodes_out = fi.N;
+    GridArgs a;
+    a.bits = fi.bits;
+    a.wpre = fi.wpre;
+    a.xyz = fi.xyz;
+    a.N = fi.N;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.slab = nz / num_p;
+    DevBuf<int32_t> dstart;
+    PLB_HIP(dstart.alloc(num_p));
+    hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p, dstart.p);
+    if (V_sub_start_out)
+        PLB_HIP(hipMemcpyAsync(V_sub_start_out, dstart, sizeof(int32_t) * num_p, hipMemcpyDeviceToHost, s));
+    if (V_out && fi.N) {
+        // Records are produced in chunks so the device copy stays bounded.
+        const int64_t chunk = 16 * 1024 * 1024;
+        DevBuf<plb_node_info> dV;
+        PLB_HIP(dV.alloc(fi.N < chunk ? fi.N : chunk));
+        for (int64_t b0 = 0; b0 < fi.N; b0 += chunk) {
+            const int64_t m = (fi.N - b0 < chunk) ? fi.N - b0 : chunk;
+            GridArgs ac = a;
+            ac.xyz = fi.xyz.p + b0;
+            ac.N = m;
+            hipLaunchKernelGGL(k_V_from_bits, dim3(nblk(m, 256)), dim3(256), 0, s, ac, dstart.p,
+                               (uint64_t*)dV.p);
+            PLB_HIP(hipGetLastError());
+            PLB_HIP(hipMemcpyAsync(V_out + b0, dV, sizeof(plb_node_info) * m, hipMemcpyDeviceToHost, s));
+            PLB_HIP(hipStreamSynchronize(s));
+        }
+    }
+    PLB_HIP(hipStreamSynchronize(s));
+    return PLB_OK;
+}
+
 extern "C" int plb_preprocess(const uint8_t* flag, int nx, int ny, int nz, int num_p, int device,
                               plb_node_info* V_out, int32_t* V_sub_start_out,
                               int64_t* n_nodes_out)
@@ -803,64 +798,14 @@ extern "C" int plb_preprocess(const uint8_t* flag, int nx, int ny, int nz, int n
     if (!flag || nx <= 0 || ny <= 0 || nz <= 0 || nx > 65535 || ny > 65535 || nz > 65535)
         return PLB_ERR_INVALID;
     if (num_p < 1 || num_p > 65535 || nz / num_p == 0) return PLB_ERR_NUM_P;
-    const int slab = nz / num_p;
-    if ((nz - 1) / slab >= num_p) return PLB_ERR_NUM_P;
+    if ((nz - 1) / (nz / num_p) >= num_p) return PLB_ERR_NUM_P;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PLB_ERR_NODEVICE;
     if (device < 0 || device >= ndev) return PLB_ERR_INVALID;
     PLB_HIP(hipSetDevice(device));
     hipStream_t s;
     PLB_HIP(hipStreamCreate(&s));
-    FlagIndex fi;
-    int rc = index_flag(flag, nx, ny, nz, s, fi);
-    if (rc == PLB_OK) {
-        if (n_nodes_out) *n_nodes_out = fi.N;
-        GridArgs a;
-        a.bits = fi.bits;
-        a.wpre = fi.wpre;
-        a.xyz = fi.xyz;
-        a.N = fi.N;
-        a.nx = nx;
-        a.ny = ny;
-        a.nz = nz;
-        a.slab = slab;
-        int32_t* dstart = nullptr;
-        uint64_t* dV = nullptr;
-        hipError_t e = hipMalloc(&dstart, sizeof(int32_t) * num_p);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_slab_starts, dim3(nblk(num_p, 64)), dim3(64), 0, s, a, num_p,
-                               dstart);
-            if (V_sub_start_out)
-                e = hipMemcpyAsync(V_sub_start_out, dstart, sizeof(int32_t) * num_p,
-                                   hipMemcpyDeviceToHost, s);
-        }
-        if (e == hipSuccess && V_out && fi.N) {
-            // Records are produced in chunks so the device copy stays bounded.
-            const int64_t chunk = 16 * 1024 * 1024;
-            e = hipMalloc(&dV, sizeof(plb_node_info) * (fi.N < chunk ? fi.N : chunk));
-            for (int64_t b0 = 0; e == hipSuccess && b0 < fi.N; b0 += chunk) {
-                const int64_t m = (fi.N - b0 < chunk) ? fi.N - b0 : chunk;
-                GridArgs ac = a;
-                ac.xyz = fi.xyz + b0;
-                ac.N = m;
-                hipLaunchKernelGGL(k_V_from_bits, dim3(nblk(m, 256)), dim3(256), 0, s, ac, dstart,
-                                   dV);
-                e = hipGetLastError();
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(V_out + b0, dV, sizeof(plb_node_info) * m,
-                                       hipMemcpyDeviceToHost, s);
-                if (e == hipSuccess) e = hipStreamSynchronize(s);
-            }
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (dV) (void)hipFree(dV);
-        if (dstart) (void)hipFree(dstart);
-        if (e != hipSuccess) {
-            plb_set_hip_error(e, "plb_preprocess", __FILE__, __LINE__);
-            rc = (e == hipErrorOutOfMemory) ? PLB_ERR_NOMEM : PLB_ERR_HIP;
-        }
-    }
-    fi.release();
+    const int rc = preprocess_on(s, flag, nx, ny, nz, num_p, V_out, V_sub_start_out, n_nodes_out);
     (void)hipStreamDestroy(s);
     return rc;
 }

@@ -117,6 +117,29 @@ struct plb_ctx {
 
 void plb_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 
+namespace plb {
+// Owner of a temporary device array: freed on every return path.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(int64_t n) { return hipMalloc((void**)&p, sizeof(T) * (size_t)(n > 0 ? n : 1)); }
+    T* release()
+    {
+        T* q = p;
+        p = nullptr;
+        return q;
+    }
+    operator T*() const { return p; }
+};
+}  // namespace plb
+
 
 // Internal entry points shared between translation units.
 namespace plb {

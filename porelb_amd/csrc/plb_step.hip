@@ -593,6 +593,10 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
 
 int launch_step(plb_ctx* c, int64_t it, bool write_macros)
 {
+    if (c->N == 0) {  // a mask without pores: nothing to update
+        c->cur ^= 1;
+        return PLB_OK;
+    }
     const StepArgs a = make_args(c, it, write_macros);
     const int bs = c->cfg.block_size;
     if (c->cfg.nodes_per_thread == 2) {

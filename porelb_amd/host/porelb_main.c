@@ -19,6 +19,8 @@
  *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick|pencil  PLB_BRICK bx,by,bz
  *   PLB_INDEX i32|tile8  PLB_BLOCK (256)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
+ *   PLB_AUTOTUNE cachefile: time layout x node order x block size x nodes per
+ *   thread on this V.bin once and reuse the result from cachefile after
  *   PLB_CHECKPOINT file: populations written after the last step
  *   PLB_RESTART file: start from such a checkpoint (fills the empty
  *   save_state/recovery_state stubs of porelb.c:367-373)
@@ -31,6 +33,7 @@
 
 #include "plb.h"
 #include "plb_files.h"
+#include "plb_tune.h"
 
 static int env_int(const char *k, int def)
 {
@@ -134,6 +137,33 @@ int main(int argc, char *argv[])
     if (save_every < 1) {
         fprintf(stderr, ">>> PLB_SAVE must be >= 1\n");
         exit(-10);
+    }
+
+    const char *tune = getenv("PLB_AUTOTUNE");
+    if (tune && *tune) {
+        /* The autotuner works on the voxel mask; V.bin's coordinates give it. */
+        const int64_t nvox = (int64_t)v.nx * v.ny * v.nz;
+        uint8_t *mask = (uint8_t *)calloc((size_t)nvox, 1);
+        if (!mask) {
+            fprintf(stderr, ">>> out of memory for the autotuner mask\n");
+            exit(-4);
+        }
+        for (int64_t k = 0; k < v.num_node; k++) {
+            const plb_node_info *r = &v.V[k];
+            if (r->x < v.nx && r->y < v.ny && r->z < v.nz)
+                mask[((int64_t)r->z * v.ny + r->y) * v.nx + r->x] = 1;
+        }
+        plb_config best;
+        rc = plb_autotune(&cfg, mask, v.nx, v.ny, v.nz, v.num_p, 20, tune, &best, NULL, 0, NULL);
+        free(mask);
+        if (rc) {
+            fprintf(stderr, ">>> autotune: %s\n", plb_strerror(rc));
+            exit(rc);
+        }
+        cfg = best;
+        printf(">>> autotune: layout %d order %d brick %d,%d,%d block %d nodes/thread %d\n", cfg.layout,
+               cfg.ordering, cfg.brick[0], cfg.brick[1], cfg.brick[2], cfg.block_size,
+               cfg.nodes_per_thread);
     }
 
     plb_ctx *c = NULL;

@@ -109,3 +109,22 @@ def test_checkpoint_resume_and_postprocess(tmp_path):
     assert np.allclose(vals[:, 2], want_rho, rtol=1e-13, atol=0)
     assert np.allclose(vals[:, 3], grid[1, :, :, 3].reshape(-1), rtol=1e-13, atol=1e-300)
     assert np.array_equal(vals[:, 6], flag[:, :, 3].reshape(-1).astype(float))
+
+
+def test_porelb_autotune_keeps_results(tmp_path):
+    """PLB_AUTOTUNE picks a launch shape (cached in the given file) and the
+    FP64 part files stay bit-identical to the reference's."""
+    d = load_tiny("tiny_d")
+    flag = d["flag"]
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    steps = int(d["steps"])
+    env = {"PLB_TAU": repr(float(d["tau"])), "PLB_GX": repr(float(d["gx"])), "PLB_PRINT": "0",
+           "PLB_AUTOTUNE": str(tmp_path / "tune.txt")}
+    for _ in range(2):  # second run reads the cache
+        r = run([os.path.join(BIN, "porelb.x"), str(steps)], tmp_path, env, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert ">>> autotune:" in r.stdout
+        assert np.array_equal(read_parts(tmp_path, 2, steps), d["np2_macros"])
+    assert len(open(tmp_path / "tune.txt").read().splitlines()) == 1
