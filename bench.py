@@ -32,6 +32,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+STORAGE_BYTES = {0: 8, 1: 4, 2: 2}   # population storage bytes per precision
 
 # Launch shapes measured on MI355X (plb_autotune candidates and a finer
 # sweep): layout 1 = AoSoA tiles of 64 nodes; index 1 = TILE8 pull table;
@@ -284,8 +285,13 @@ def main():
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
     avg_ms = main_run["event_ms"] / steps
+    # Algorithmic bytes per pore-node update: both population copies plus a
+    # 19-entry int32 pull table, B(s) = 2*19*s + 19*4 (228 B for FP32 storage).
+    # The layout's own bytes (TILE8 index, 18 entries) are reported beside it.
+    algo_bpu = 2 * 19 * STORAGE_BYTES[w["precision"]] + 19 * 4
     bpu = main_run["bytes_per_update"]
-    achieved = bpu * n / (avg_ms * 1e-3) / 1e9
+    achieved = algo_bpu * n / (avg_ms * 1e-3) / 1e9
+    layout_gbs = bpu * n / (avg_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(REPO, "profiles", f"step_{args.workload}_pmc.md")
     if os.path.exists(tfile):   # written from two rocprofv3 --pmc passes
@@ -301,7 +307,7 @@ def main():
                          fp["ordering"], fp["block"], fp["index"], brick)
         fp64_mlups = fp64["nodes"] * max(20, steps // 2) / fp64["wall_s"] / 1e6
         secondary = {"fp64_plain_mlups": round(fp64_mlups, 1),
-                     "fp64_plain_roofline_frac": round(fp64["bytes_per_update"] * fp64["nodes"] /
+                     "fp64_plain_roofline_frac": round((2 * 19 * 8 + 19 * 4) * fp64["nodes"] /
                                                        (fp64["event_ms"] / max(20, steps // 2) * 1e-3) /
                                                        1e9 / HBM_PEAK_GBS, 4),
                      "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3),
@@ -328,9 +334,12 @@ def main():
                    "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "mlups_vs_228B_roofline": round(mlups / (HBM_PEAK_GBS * 1e9 / 228.0 / 1e6), 4),
-                     "bytes_per_update": bpu, "kernel_ms": round(avg_ms, 4),
-                     "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "algorithmic_bytes_per_update": algo_bpu, "kernel_ms": round(avg_ms, 4),
+                     "layout_bytes_per_update": round(bpu, 3), "layout_gbs": round(layout_gbs, 1),
+                     "measured_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "copy_ceiling_gbs": round(copy_gbs, 1),
+                     "measured_frac_of_copy": (round(traffic / (avg_ms * 1e-3) / 1e9 / copy_gbs, 4)
+                                               if traffic else None)},
         "cpu_baseline": cpu,
         "secondary": secondary,
         "K": main_run["reduction"]["K"],

@@ -4,6 +4,7 @@
  *
  *     ./postprocess.x A t1 t2 ...        full field  -> full_TTTTTTT.dat
  *     ./postprocess.x X|Y|Z n t1 t2 ...  one slice   -> Slice_[XYZ]_TTTTTTT.dat
+ *     ./postprocess.x K t1 t2 ...        permeability report on stdout
  *
  * Reads V.bin, flag.bin and every rank's part_PPP_TTTTTTT.bin and scatters
  * the macros to the voxel grid by V's coordinates (postprocess.c:160-214).
@@ -11,6 +12,13 @@
  * column is printed as a number, where the reference passes an int to %e).
  * `A` writes the same variables "X Y Z rho vx vy vz flag" as an ASCII
  * Tecplot BLOCK zone instead of TECIO's binary file.
+ *
+ * `K` is the permeability report the reference leaves to getK.c's analytic
+ * estimate: from the part files alone it prints, per time, the FP64 sums
+ * over all pore nodes in V.bin order, <u_x> = sum u_x / N, the superficial
+ * velocity q = sum u_x / (nx ny nz) and K = nu q / g_x with
+ * nu = (tau - 1/2)/3 (porelb.c:89).  tau and g_x come from PLB_TAU and
+ * PLB_GX, with porelb.x's defaults (the reference's porelb.c:83-94).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -28,6 +36,39 @@ static void print_help(void)
     printf("Usage: ./postprocess.x  [A|X|Y|Z] [n] time_list\n");
     printf("Output 3D full data     : ./postprocess.x A time_list\n");
     printf("Output slice at  X = 20 : ./postprocess.x X 20  time_list\n");
+    printf("Permeability report     : ./postprocess.x K time_list\n");
+}
+
+static double env_double(const char *name, double dflt)
+{
+    const char *s = getenv(name);
+    return (s && *s) ? atof(s) : dflt;
+}
+
+/* Sums over every rank's part file for time t, in V.bin record order. */
+static void report_K(const plb_vfile *V, int t, double tau, double gx)
+{
+    double s_rho = 0.0, s_ux = 0.0;
+    for (int p = 0; p < V->num_p; p++) {
+        const long b = V->V_sub_start[p];
+        const long N = (p + 1 < V->num_p ? V->V_sub_start[p + 1] : V->num_node) - b;
+        double *buf = malloc(sizeof(double) * 4 * (N ? N : 1));
+        if (!buf || plb_read_part(p, t, N, buf, buf + N, buf + 2 * N, buf + 3 * N)) {
+            fprintf(stderr, "Part file openning error!\n");
+            exit(-1);
+        }
+        for (long k = 0; k < N; k++) {
+            s_rho += buf[k];
+            s_ux += buf[N + k];
+        }
+        free(buf);
+    }
+    const double nvox = (double)V->nx * V->ny * V->nz;
+    const double mean = V->num_node ? s_ux / V->num_node : 0.0;
+    const double q = s_ux / nvox;
+    const double nu = 1.0 / 3.0 * (tau - 0.5); /* porelb.c:89 */
+    printf("t= %d N= %d sum_rho= %.17e sum_ux= %.17e mean_ux= %.17e q= %.17e K= %.17e\n", t,
+           V->num_node, s_rho, s_ux, mean, q, nu * q / gx);
 }
 
 static void slice(char axis, int n, int t)
@@ -102,12 +143,13 @@ int main(int argc, char *argv[])
         exit(-2);
     }
     const char mode = argv[1][0];
-    if (strlen(argv[1]) != 1 || (mode != 'A' && mode != 'X' && mode != 'Y' && mode != 'Z')) {
+    if (strlen(argv[1]) != 1 ||
+        (mode != 'A' && mode != 'X' && mode != 'Y' && mode != 'Z' && mode != 'K')) {
         fprintf(stderr, "Not implemented yet!\n");
         exit(-3);
     }
-    int first = mode == 'A' ? 2 : 3, n_slice = 0;
-    if (mode != 'A') {
+    int first = (mode == 'A' || mode == 'K') ? 2 : 3, n_slice = 0;
+    if (first == 3) {
         if (argc < 3) {
             print_help();
             exit(-2);
@@ -124,6 +166,13 @@ int main(int argc, char *argv[])
     nx = V.nx;
     ny = V.ny;
     nz = V.nz;
+    if (mode == 'K') {
+        const double tau = env_double("PLB_TAU", 0.8), gx = env_double("PLB_GX", 1e-3);
+        printf("Total %d time seriese data\n", argc - first);
+        for (int a = first; a < argc; a++) report_K(&V, atoi(argv[a]), tau, gx);
+        plb_free_vfile(&V);
+        return 0;
+    }
     const long size = (long)nx * ny * nz;
     if ((mode == 'X' && (n_slice < 0 || n_slice >= nx)) || (mode == 'Y' && (n_slice < 0 || n_slice >= ny)) ||
         (mode == 'Z' && (n_slice < 0 || n_slice >= nz))) {

@@ -55,3 +55,62 @@ def test_mkflag_writes_mask(tmp_path, plb):
     assert r.returncode == 0, r.stderr
     got = np.fromfile(tmp_path / "m.bin", dtype=np.uint8).reshape(8, 12, 16)
     assert np.array_equal(got, plb.mask_noise((8, 12, 16), 2.0, 0.4, 5))
+
+
+def test_getK_matches_oracle_and_table(tmp_path):
+    """getK.x prints the Sangani-Acrivos K1 and Kozeny-Carman K2 of getK.c:39-72
+    (pinned through the oracle's BCC tests); the table block carries the extra
+    porosity factor of getK.c:85."""
+    import oracle as O
+
+    for chi, nx in [(0.2, 33), (0.5, 65), (0.85, 129)]:
+        r = run([os.path.join(BIN, "getK.x"), repr(chi), str(nx)], tmp_path)
+        assert r.returncode == 0, r.stderr
+        got = dict(line.split("=") for line in r.stdout.split())
+        want = O.getK_bcc(chi, nx)
+        assert float(got["K1"]) == pytest.approx(want["K1"], rel=1e-6)
+        assert float(got["K2"]) == pytest.approx(want["K2"], rel=1e-6)
+    r = run([os.path.join(BIN, "getK.x"), "0.5", "65", "table"], tmp_path)
+    rows = [list(map(float, ln.split("\t"))) for ln in r.stdout.splitlines()[3:]]
+    assert len(rows) == 31
+    for c, eps, k1, k2 in rows[1:]:
+        want = O.getK_bcc(c, 65)
+        assert eps == pytest.approx(want["eps"], rel=1e-6)
+        assert k1 == pytest.approx(want["K1"] * want["eps"], rel=1e-6)
+        assert k2 == pytest.approx(want["K2"], rel=1e-6)
+    r = run([os.path.join(BIN, "getK.x"), "0.5"], tmp_path)
+    assert r.returncode == (-1) & 0xFF and "argc err" in r.stderr
+
+
+def test_postprocess_permeability_report(tmp_path, plb):
+    """postprocess.x K sums the part files in V.bin order in FP64 and reports
+    <u_x>, q = sum u_x / (nx ny nz) and K = (tau - 1/2)/3 * q / g_x."""
+    import oracle as O
+
+    flag = (np.random.default_rng(4).random((6, 5, 7)) < 0.55).astype(np.uint8)
+    V, start = O.build_V(flag, 2)
+    plb.write_vbin(str(tmp_path / "V.bin"), V, start, flag.shape)
+    rng = np.random.default_rng(5)
+    bounds = list(start) + [len(V)]
+    parts = []
+    for p in range(2):
+        n = bounds[p + 1] - bounds[p]
+        m = rng.standard_normal((4, n)) * 1e-3
+        m[0] += 1.0
+        m.tofile(tmp_path / f"part_{p:03d}_0000070.bin")
+        parts.append(m)
+    r = run([os.path.join(BIN, "postprocess.x"), "K", "70"], tmp_path, {"PLB_TAU": "0.9", "PLB_GX": "2e-5"})
+    assert r.returncode == 0, r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("t= 70")][0].split()
+    got = {line[i].rstrip("="): line[i + 1] for i in range(0, len(line), 2)}
+    s_rho = s_ux = 0.0
+    for m in parts:             # sequential order, as the C loop adds
+        for a, b in zip(m[0], m[1]):
+            s_rho += float(a)
+            s_ux += float(b)
+    assert int(got["N"]) == len(V)
+    assert float(got["sum_rho"]) == s_rho and float(got["sum_ux"]) == s_ux
+    assert float(got["mean_ux"]) == s_ux / len(V)
+    q = s_ux / flag.size
+    assert float(got["q"]) == q
+    assert float(got["K"]) == 1.0 / 3.0 * (0.9 - 0.5) * q / 2e-5

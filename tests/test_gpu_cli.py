@@ -128,3 +128,26 @@ def test_porelb_autotune_keeps_results(tmp_path):
         assert ">>> autotune:" in r.stdout
         assert np.array_equal(read_parts(tmp_path, 2, steps), d["np2_macros"])
     assert len(open(tmp_path / "tune.txt").read().splitlines()) == 1
+
+
+def test_postprocess_K_matches_device_reduction(tmp_path, plb):
+    """postprocess.x K on porelb.x's part files agrees with the device FP64
+    reduction of the same run (summation order differs: 1e-12)."""
+    flag = plb.mask_spheres((24, 20, 28), 2.0, 4.0, 0.4, 3)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-5", "PLB_PRINT": "0"}
+    assert run([os.path.join(BIN, "porelb.x"), "300"], tmp_path, env).returncode == 0
+    r = run([os.path.join(BIN, "postprocess.x"), "K", "300"], tmp_path, env)
+    assert r.returncode == 0, r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("t= 300")][0].split()
+    got = {line[i].rstrip("="): float(line[i + 1]) for i in range(0, len(line), 2)}
+    vb = O.read_vbin(str(tmp_path / "V.bin"))
+    with plb.Lattice.from_V(vb["V"], vb["start"], vb["dims"], tau=0.9, gx=1e-5) as lat:
+        lat.step(0, 300)
+        red = lat.reduce()
+    assert got["N"] == red["n_nodes"]
+    for k in ("sum_ux", "mean_ux", "q", "K"):
+        assert got[k] == pytest.approx(red[k], rel=1e-12), k
+    assert got["K"] > 0
