@@ -77,8 +77,12 @@ enum { PLB_ORDER_REFERENCE = 0, PLB_ORDER_BRICK = 1, PLB_ORDER_PENCIL = 2 };
 /* Pull-table encoding on the device: I32 = one int32 per (node, direction);
  * TILE8 = per 64-node tile and direction one int32 base plus a u8 offset per
  * node (254 = constant, 255 = bounce), with int32 entries for the rare
- * (tile, direction) pairs whose sources span more than 253 ids. */
-enum { PLB_INDEX_I32 = 0, PLB_INDEX_TILE8 = 1 };
+ * (tile, direction) pairs whose sources span more than 253 ids;
+ * BITMAP = no table: one pore bit per voxel in node order with the pore count
+ * before every 32-voxel word, plus each node's voxel position, from which the
+ * step kernel derives every source (half-way bounce-back only; node order
+ * REFERENCE, or PENCIL with brick[0] >= nx and brick[2] == 1). */
+enum { PLB_INDEX_I32 = 0, PLB_INDEX_TILE8 = 1, PLB_INDEX_BITMAP = 2 };
 
 typedef struct {
     double tau;          /* relaxation time (reference: 0.8, porelb.c:87)         */
@@ -87,12 +91,12 @@ typedef struct {
     int precision;       /* PLB_FP64 | PLB_FP32 | PLB_FP16                        */
     int bb_mode;         /* PLB_BB_REFCOMPAT | PLB_BB_HALFWAY                     */
     int layout;          /* PLB_LAYOUT_SOA | PLB_LAYOUT_AOSOA                     */
-    int ordering;        /* PLB_ORDER_REFERENCE | PLB_ORDER_BRICK                 */
-    int brick[3];        /* brick edge lengths x,y,z for PLB_ORDER_BRICK          */
+    int ordering;        /* PLB_ORDER_REFERENCE | PLB_ORDER_BRICK | PLB_ORDER_PENCIL */
+    int brick[3];        /* brick edge lengths x,y,z for PLB_ORDER_BRICK / PENCIL */
     int block_size;      /* threads per block of the step kernel (64..1024)       */
     int device;          /* HIP device ordinal                                    */
     int fp16_scale_log2; /* FP16: stored = (f - w rho0) * 2^k; 0 = derive from gx */
-    int index_kind;      /* PLB_INDEX_I32 | PLB_INDEX_TILE8                       */
+    int index_kind;      /* PLB_INDEX_I32 | PLB_INDEX_TILE8 | PLB_INDEX_BITMAP    */
     int nodes_per_thread; /* 1, or 2: a wave advances two 64-node tiles at once
                              (block_size <= 512; single lattices only)          */
 } plb_config;

@@ -30,7 +30,7 @@ FP64, FP32, FP16 = 0, 1, 2
 BB_REFCOMPAT, BB_HALFWAY = 0, 1
 LAYOUT_SOA, LAYOUT_AOSOA = 0, 1
 ORDER_REFERENCE, ORDER_BRICK, ORDER_PENCIL = 0, 1, 2
-INDEX_I32, INDEX_TILE8 = 0, 1
+INDEX_I32, INDEX_TILE8, INDEX_BITMAP = 0, 1, 2
 TYPE_FLUID, TYPE_SOLID, TYPE_EXTERN = 0, 1, 2
 ERR_DENSITY = -6
 

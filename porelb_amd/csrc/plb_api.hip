@@ -88,7 +88,16 @@ int plb::check_cfg(const plb_config* cfg)
         return PLB_ERR_INVALID;
     if (cfg->block_size < 64 || cfg->block_size > 1024 || cfg->block_size % 64) return PLB_ERR_INVALID;
     if (cfg->fp16_scale_log2 < 0 || cfg->fp16_scale_log2 > 100) return PLB_ERR_INVALID;
-    if (cfg->index_kind != PLB_INDEX_I32 && cfg->index_kind != PLB_INDEX_TILE8) return PLB_ERR_INVALID;
+    if (cfg->index_kind != PLB_INDEX_I32 && cfg->index_kind != PLB_INDEX_TILE8 &&
+        cfg->index_kind != PLB_INDEX_BITMAP)
+        return PLB_ERR_INVALID;
+    // BITMAP derives sources from voxel positions: half-way bounce-back, and
+    // node orders that are whole x-rows (checked against nx at build time).
+    if (cfg->index_kind == PLB_INDEX_BITMAP &&
+        (cfg->bb_mode != PLB_BB_HALFWAY ||
+         (cfg->ordering != PLB_ORDER_REFERENCE && cfg->ordering != PLB_ORDER_PENCIL) ||
+         (cfg->ordering == PLB_ORDER_PENCIL && cfg->brick[2] != 1)))
+        return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread != 1 && cfg->nodes_per_thread != 2) return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread == 2 && cfg->block_size > 512) return PLB_ERR_INVALID;
     return PLB_OK;
@@ -150,7 +159,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
     }
-    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->twide, c->perm,
+    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->twide, c->bm, c->vox, c->perm,
                     c->inv_perm, c->node_xyz, c->macros, c->fault, c->red};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

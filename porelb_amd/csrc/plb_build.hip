@@ -460,6 +460,7 @@ int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref)
     PLB_HIP(hipStreamSynchronize(s));
     c->index_bytes = (Q - 1) * 4.0;
     if (c->cfg.index_kind == PLB_INDEX_TILE8) return compress_tile8(c);
+    if (c->cfg.index_kind == PLB_INDEX_BITMAP) return build_bitmap(c);
     return PLB_OK;
 }
 
@@ -579,6 +580,109 @@ int compress_tile8(plb_ctx* c)
     c->index_bytes = c->N ? ((double)T * TILE8_WORDS * 256 + (double)nk * 4 + (double)c->n_wide * 256) /
                                 (double)c->N
                           : 0.0;
+    return PLB_OK;
+}
+
+// ---------------------------------------------------------------- BITMAP index
+// Node order as a voxel position: pencils of py rows, z fastest across
+// pencils, V = x + nx*(ly + py*(z + nz*yb)) with y = yb*py + ly.  With
+// py = ny this is the V-file order z,y,x.  Node ids are the ranks of the
+// pores in this order, so id(v) = pores before v's 32-voxel word + pores
+// below v inside the word.
+struct PencilMap {
+    uint32_t nx, nz, py;
+};
+
+__global__ void k_bm_bits(const uint64_t* __restrict__ xyz, int64_t N, PencilMap m, uint32_t* bits,
+                          uint32_t* vox)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    int x, y, z;
+    unpack(xyz[g], x, y, z);
+    const uint32_t yb = (uint32_t)y / m.py, ly = (uint32_t)y - yb * m.py;
+    const uint32_t v = ((yb * m.nz + (uint32_t)z) * m.py + ly) * m.nx + (uint32_t)x;
+    vox[g] = v;
+    atomicOr(bits + (v >> 5), 1u << (v & 31u));
+}
+
+__global__ void k_bm_count(const uint32_t* bits, int64_t nw, uint64_t* cnt)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) cnt[w] = (uint64_t)__popc(bits[w]);
+}
+
+__global__ void k_bm_pack(const uint32_t* bits, const uint64_t* pre, int64_t nw, uint2* bm)
+{
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) bm[w] = make_uint2(bits[w], (uint32_t)pre[w]);
+}
+
+// Every node must find itself: its voxel is a pore whose rank is its id.
+__global__ void k_bm_check(const uint2* bm, const uint32_t* vox, int64_t N, int* err)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const uint32_t v = vox[g];
+    const uint2 wd = bm[v >> 5];
+    const uint32_t b = v & 31u;
+    const uint32_t id = wd.y + (uint32_t)__popc(wd.x & ((1u << b) - 1u));
+    if (!((wd.x >> b) & 1u) || (int64_t)id != g) *err = 1;
+}
+
+// Replaces the int32 table of a context by the BITMAP index.
+int build_bitmap(plb_ctx* c)
+{
+    hipStream_t s = c->stream;
+    const plb_config& cfg = c->cfg;
+    uint32_t py = (uint32_t)c->ny;
+    if (cfg.ordering == PLB_ORDER_PENCIL) {
+        if (cfg.brick[0] < c->nx || cfg.brick[2] != 1) return PLB_ERR_INVALID;
+        py = (uint32_t)std::min(cfg.brick[1], c->ny);
+    } else if (cfg.ordering != PLB_ORDER_REFERENCE) {
+        return PLB_ERR_INVALID;
+    }
+    const uint64_t nyb = ((uint64_t)c->ny + py - 1) / py;
+    const uint64_t nvox = (uint64_t)c->nx * nyb * py * (uint64_t)c->nz;
+    if (nvox > 0xffffffffull) return PLB_ERR_RANGE;
+    const int64_t nw = (int64_t)((nvox + 31) / 32);
+    DevBuf<uint32_t> bits;
+    DevBuf<uint64_t> cnt, pre;
+    DevBuf<int> derr;
+    PLB_HIP(bits.alloc(nw));
+    PLB_HIP(cnt.alloc(nw));
+    PLB_HIP(pre.alloc(nw));
+    PLB_HIP(derr.alloc(1));
+    PLB_HIP(hipMemsetAsync(bits, 0, sizeof(uint32_t) * nw, s));
+    PLB_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+    PLB_HIP(hipMalloc(&c->vox, sizeof(uint32_t) * c->Npad));
+    PLB_HIP(hipMemsetAsync(c->vox, 0, sizeof(uint32_t) * c->Npad, s));
+    PLB_HIP(hipMalloc(&c->bm, sizeof(uint2) * nw));
+    PencilMap m;
+    m.nx = (uint32_t)c->nx;
+    m.nz = (uint32_t)c->nz;
+    m.py = py;
+    if (c->N > 0)
+        hipLaunchKernelGGL(k_bm_bits, dim3(nblk(c->N, 256)), dim3(256), 0, s, c->node_xyz, c->N, m,
+                           bits.p, c->vox);
+    hipLaunchKernelGGL(k_bm_count, dim3(nblk(nw, 256)), dim3(256), 0, s, bits.p, nw, cnt.p);
+    PLB_HIP(hipGetLastError());
+    int rc = exclusive_scan_u64(cnt, pre, nw, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bm_pack, dim3(nblk(nw, 256)), dim3(256), 0, s, bits.p, pre.p, nw, c->bm);
+    if (c->N > 0)
+        hipLaunchKernelGGL(k_bm_check, dim3(nblk(c->N, 256)), dim3(256), 0, s, c->bm, c->vox, c->N,
+                           derr.p);
+    PLB_HIP(hipGetLastError());
+    int herr = 0;
+    PLB_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipStreamSynchronize(s));
+    if (herr) return PLB_ERR_INVALID;  // node order and voxel order disagree
+    (void)hipFree(c->tab);
+    c->tab = nullptr;
+    c->py = (int)py;
+    c->n_bm_words = nw;
+    c->index_bytes = c->N ? (4.0 * (double)c->N + 8.0 * (double)nw) / (double)c->N : 0.0;
     return PLB_OK;
 }
 

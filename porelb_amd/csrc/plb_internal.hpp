@@ -67,6 +67,29 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
     return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Division of a u32 by a run-time constant d (1 <= d < 2^31) with one
+// high multiply: l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(l,1))) >> max(l-1,0) with t = mulhi(m, n)
+// (Granlund & Montgomery, PLDI 1994, section 4).
+struct FastDiv {
+    uint32_t m = 1, s1 = 0, s2 = 0;
+};
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.s1 = l > 0 ? 1 : 0;
+    f.s2 = l > 0 ? l - 1 : 0;
+    return f;
+}
+__device__ __forceinline__ uint32_t fastdiv(uint32_t n, const FastDiv& f)
+{
+    const uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 __host__ __device__ __forceinline__ uint64_t pack_xyz(uint64_t x, uint64_t y, uint64_t z)
 {
     return x | (y << 16) | (z << 32);
@@ -90,6 +113,10 @@ struct plb_ctx {
     uint32_t* toff = nullptr;
     int32_t* twide = nullptr;
     int64_t n_wide = 0;
+    uint2* bm = nullptr;        // BITMAP index: {pore bits, pores before} per 32 voxels
+    uint32_t* vox = nullptr;    // BITMAP index: node -> voxel position in node order
+    int py = 0;                 // BITMAP index: rows per pencil (ny for the V order)
+    int64_t n_bm_words = 0;
     double index_bytes = 72.0;  // pull-table bytes per node
     uint32_t* perm = nullptr;   // internal -> V order (null: identity)
     uint32_t* inv_perm = nullptr;
@@ -165,6 +192,7 @@ int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int
 int check_cfg(const plb_config* cfg);
 int fp16_scale_log2_for(double gx);
 int compress_tile8(plb_ctx* c);
+int build_bitmap(plb_ctx* c);
 int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, const uint64_t* xyz_ref,
                   hipStream_t s, uint32_t** inv_perm, uint32_t** perm, uint64_t* xyz_int);
 int permute_ref_table(const int32_t* tabref, int64_t n, const uint32_t* inv_perm, int32_t* out,

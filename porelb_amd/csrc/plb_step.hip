@@ -41,6 +41,11 @@ struct StepArgs {
     unsigned long long* fault;
     const uint32_t* perm;
     float scale, inv_scale;
+    // BITMAP index: voxel position V = x + nx*(ly + py*(z + nz*yb)), y = yb*py + ly
+    const uint2* bm;
+    const uint32_t* vox;
+    uint32_t nx, ny, nz, py;
+    FastDiv div_nx, div_py, div_nz;
 };
 
 // ---------------------------------------------------------------- FP64, reference order
@@ -85,7 +90,51 @@ struct Entries {
     int32_t t[Q];                // I32: raw entries; after decode: decoded
     uint32_t w[TILE8_WORDS];     // TILE8: packed offsets of this lane
     int32_t desc[Q];             // TILE8: the tile's descriptors (uniform)
+    uint32_t v;                  // BITMAP: the node's voxel position
 };
+
+// ---------------------------------------------------------------- BITMAP sources
+// Periodic neighbour coordinate c - e (e in {-1, 0, 1}) on [0, n).
+__device__ __forceinline__ uint32_t wrap_sub(uint32_t c, int e, uint32_t n)
+{
+    if (e > 0) return c == 0 ? n - 1 : c - 1;
+    if (e < 0) return c + 1 == n ? 0 : c + 1;
+    return c;
+}
+
+// Voxel position of the pull source x - e_j, for any node.
+__device__ __forceinline__ uint32_t bitmap_src_general(const StepArgs& a, int j, uint32_t x, uint32_t y,
+                                                       uint32_t z)
+{
+    const uint32_t xs = wrap_sub(x, EX[j], a.nx);
+    const uint32_t ys = wrap_sub(y, EY[j], a.ny);
+    const uint32_t zs = wrap_sub(z, EZ[j], a.nz);
+    const uint32_t yb = fastdiv(ys, a.div_py);
+    const uint32_t ly = ys - yb * a.py;
+    return ((yb * a.nz + zs) * a.py + ly) * a.nx + xs;
+}
+
+__device__ __forceinline__ void bitmap_coords(const StepArgs& a, uint32_t v, uint32_t& x, uint32_t& y,
+                                              uint32_t& z, uint32_t& ly)
+{
+    const uint32_t r1 = fastdiv(v, a.div_nx);
+    x = v - r1 * a.nx;
+    const uint32_t r2 = fastdiv(r1, a.div_py);
+    ly = r1 - r2 * a.py;
+    const uint32_t yb = fastdiv(r2, a.div_nz);
+    z = r2 - yb * a.nz;
+    y = yb * a.py + ly;
+}
+
+// Node id of the pore at voxel position v, or TAB_BOUNCE for a solid voxel
+// (half-way bounce-back): the word's pore count plus the pores below v.
+__device__ __forceinline__ int32_t bitmap_lookup(uint2 wd, uint32_t v)
+{
+    const uint32_t b = v & 31u;
+    const uint32_t below = __builtin_amdgcn_ubfe(wd.x, 0, b);
+    const int32_t id = (int32_t)(wd.y + __builtin_popcount(below));
+    return ((wd.x >> b) & 1u) ? id : TAB_BOUNCE;
+}
 
 template <int LAYOUT, int IDX>
 __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
@@ -93,6 +142,8 @@ __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entr
     if constexpr (IDX == PLB_INDEX_I32) {
 #pragma unroll
         for (int j = 1; j < Q; j++) e.t[j] = a.tab[tpos<LAYOUT>(j - 1, i, a.Npad)];
+    } else if constexpr (IDX == PLB_INDEX_BITMAP) {
+        e.v = a.vox[i];
     } else {
         // TILE8: the wave is one 64-node tile, so its 18 descriptors are
         // uniform (scalar loads); each lane reads 5 words of packed offsets.
@@ -109,7 +160,36 @@ __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entr
 template <int IDX>
 __device__ __forceinline__ void decode_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
 {
-    if constexpr (IDX == PLB_INDEX_TILE8) {
+    if constexpr (IDX == PLB_INDEX_BITMAP) {
+        const uint32_t v = e.v;
+        uint32_t x, y, z, ly;
+        bitmap_coords(a, v, x, y, z, ly);
+        // Away from pencil edges and the periodic y/z boundary every source is
+        // a fixed offset in node order; x wraps are a per-lane correction.
+        const bool inner = (int)ly >= 1 && (int)ly <= (int)a.py - 2 && (int)y >= 1 &&
+                           (int)y <= (int)a.ny - 2 && (int)z >= 1 && (int)z <= (int)a.nz - 2;
+        uint32_t vs[Q];
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+            const uint32_t xm = x == 0 ? a.nx : 0u;            // source at x-1 wraps to nx-1
+            const uint32_t xp = x + 1 == a.nx ? a.nx : 0u;     // source at x+1 wraps to 0
+            const uint32_t row = a.nx, plane = a.nx * a.py;
+#pragma unroll
+            for (int j = 1; j < Q; j++) {
+                uint32_t d = v - (uint32_t)EX[j] - (uint32_t)EY[j] * row - (uint32_t)EZ[j] * plane;
+                if (EX[j] > 0) d += xm;
+                if (EX[j] < 0) d -= xp;
+                vs[j] = d;
+            }
+        } else {
+#pragma unroll
+            for (int j = 1; j < Q; j++) vs[j] = bitmap_src_general(a, j, x, y, z);
+        }
+        uint2 wd[Q];
+#pragma unroll
+        for (int j = 1; j < Q; j++) wd[j] = a.bm[vs[j] >> 5];
+#pragma unroll
+        for (int j = 1; j < Q; j++) e.t[j] = bitmap_lookup(wd[j], vs[j]);
+    } else if constexpr (IDX == PLB_INDEX_TILE8) {
         int32_t any = 0;
 #pragma unroll
         for (int j = 1; j < Q; j++) {
@@ -231,18 +311,26 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     const float omega = (float)a.omega;
     const float c1 = 1.0f - omega;
     const float c2 = (1.0f - 0.5f * omega) * 3.0f * F;
-    const float usq = 1.5f * (u * u + v * v + w * w);
+    // Fused multiply-adds are written out (the library is built without
+    // contraction for the FP64 path's sake); this path is held to a
+    // tolerance against FP64, not to bitwise equality.
+    const float usq = 1.5f * fmaf(u, u, fmaf(v, v, w * w));
+    const float fu = -c2 * u;  // force term c2 (e_jx - u) for e_jx = 0, then +-c2
     const float sc = a.scale;
+    S* __restrict__ onode = fout + fpos<LAYOUT>(0, i, a.Npad);
 #pragma unroll
     for (int j = 0; j < Q; j++) {
-        const float eu = (float)EX[j] * u + (float)EY[j] * v + (float)EZ[j] * w;
-        const float P = eu * (3.0f + 4.5f * eu) - usq;
+        float eu = 0.f;
+        if (EX[j]) eu = EX[j] > 0 ? u : -u;
+        if (EY[j]) eu += EY[j] > 0 ? v : -v;
+        if (EZ[j]) eu += EZ[j] > 0 ? w : -w;
+        const float P = fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
         const float wj = (float)TP[j];
-        const float dfeq = wj * (drho + rho * P);
-        const float feq = wj * rho0 + dfeq;
-        const float frc = c2 * ((float)EX[j] - u);
-        const float o = c1 * d[j] + omega * dfeq + frc * feq;
-        st_dev(fout, fpos<LAYOUT>(j, i, a.Npad), o, sc);
+        const float dfeq = wj * fmaf(rho, P, drho);
+        const float feq = fmaf(wj, rho0, dfeq);
+        const float frc = EX[j] > 0 ? fu + c2 : (EX[j] < 0 ? fu - c2 : fu);
+        const float o = fmaf(c1, d[j], fmaf(omega, dfeq, frc * feq));
+        st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
     }
 }
 
@@ -412,6 +500,13 @@ __global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const u
         Entries<IDX> e;
         issue_entries<LAYOUT, IDX>(a, k, e);
         for (int j = 1; j < Q; j++) t[j] = e.t[j];
+    } else if constexpr (IDX == PLB_INDEX_BITMAP) {
+        uint32_t x, y, z, ly;
+        bitmap_coords(a, a.vox[k], x, y, z, ly);
+        for (int j = 1; j < Q; j++) {
+            const uint32_t vs = bitmap_src_general(a, j, x, y, z);
+            t[j] = bitmap_lookup(a.bm[vs >> 5], vs);
+        }
     } else {
         // Lanes of a wave hold unrelated nodes here: decode without the
         // per-wave descriptor shortcut.
@@ -540,31 +635,33 @@ static unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
         }                                                                                   \
     } while (0)
 
-#define PLB_DISPATCH3(PREC, LAYOUT, IDX, KERNEL, ...)                                        \
-    do {                                                                                     \
-        if ((IDX) == PLB_INDEX_I32) {                                                        \
-            if ((PREC) == PLB_FP64) {                                                        \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
-            } else if ((PREC) == PLB_FP32) {                                                 \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
-            } else {                                                                         \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA, PLB_INDEX_I32>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), __VA_ARGS__); \
-            }                                                                                \
-        } else {                                                                             \
-            if ((PREC) == PLB_FP64) {                                                        \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-            } else if ((PREC) == PLB_FP32) {                                                 \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-            } else {                                                                         \
-                if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-                else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), __VA_ARGS__); \
-            }                                                                                \
-        }                                                                                    \
+#define PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, IDXC, KERNEL, ...)                                  \
+    do {                                                                                        \
+        if ((PREC) == PLB_FP64) {                                                               \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_SOA, IDXC>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<double, PLB_LAYOUT_AOSOA, IDXC>), __VA_ARGS__);      \
+        } else if ((PREC) == PLB_FP32) {                                                        \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_SOA, IDXC>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<float, PLB_LAYOUT_AOSOA, IDXC>), __VA_ARGS__);       \
+        } else {                                                                                \
+            if ((LAYOUT) == PLB_LAYOUT_SOA) hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_SOA, IDXC>), __VA_ARGS__); \
+            else hipLaunchKernelGGL((KERNEL<__half, PLB_LAYOUT_AOSOA, IDXC>), __VA_ARGS__);      \
+        }                                                                                       \
+    } while (0)
+
+// All three index kinds (single lattice).
+#define PLB_DISPATCH3(PREC, LAYOUT, IDX, KERNEL, ...)                                         \
+    do {                                                                                      \
+        if ((IDX) == PLB_INDEX_I32) PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, PLB_INDEX_I32, KERNEL, __VA_ARGS__); \
+        else if ((IDX) == PLB_INDEX_TILE8) PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, PLB_INDEX_TILE8, KERNEL, __VA_ARGS__); \
+        else PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, PLB_INDEX_BITMAP, KERNEL, __VA_ARGS__);        \
+    } while (0)
+
+// Table index kinds only (batches of samples).
+#define PLB_DISPATCH2(PREC, LAYOUT, IDX, KERNEL, ...)                                         \
+    do {                                                                                      \
+        if ((IDX) == PLB_INDEX_I32) PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, PLB_INDEX_I32, KERNEL, __VA_ARGS__); \
+        else PLB_DISPATCH_ONE_IDX(PREC, LAYOUT, PLB_INDEX_TILE8, KERNEL, __VA_ARGS__);         \
     } while (0)
 
 static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
@@ -588,6 +685,15 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.perm = c->perm;
     a.scale = c->scale;
     a.inv_scale = c->inv_scale;
+    a.bm = c->bm;
+    a.vox = c->vox;
+    a.nx = (uint32_t)c->nx;
+    a.ny = (uint32_t)c->ny;
+    a.nz = (uint32_t)c->nz;
+    a.py = (uint32_t)(c->py > 0 ? c->py : c->ny);
+    a.div_nx = make_fastdiv(a.nx);
+    a.div_py = make_fastdiv(a.py);
+    a.div_nz = make_fastdiv(a.nz);
     return a;
 }
 
@@ -626,7 +732,7 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
         b.sample_end = sample_end;
         b.omega = omega;
         b.gx = gx;
-        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step_batch,
+        PLB_DISPATCH2(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step_batch,
                       dim3(n_active), dim3(c->cfg.block_size), 0, c->stream, a, b);
         PLB_HIP(hipGetLastError());
     }
@@ -670,25 +776,23 @@ int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* d
     return PLB_OK;
 }
 
+#define PLB_READ_TABLE(LAYOUT, IDXC)                                                       \
+    hipLaunchKernelGGL((k_read_table<LAYOUT, IDXC>), g, b, 0, c->stream, a, c->inv_perm, c->perm, \
+                       first, n, dout)
+
 int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout)
 {
     const StepArgs a = make_args(c, 0, false);
     const dim3 g(nblk(n, 256)), b(256);
-    const bool t8 = c->cfg.index_kind == PLB_INDEX_TILE8;
+    const int idx = c->cfg.index_kind;
     if (c->cfg.layout == PLB_LAYOUT_SOA) {
-        if (t8)
-            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_SOA, PLB_INDEX_TILE8>), g, b, 0, c->stream, a,
-                               c->inv_perm, c->perm, first, n, dout);
-        else
-            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_SOA, PLB_INDEX_I32>), g, b, 0, c->stream, a,
-                               c->inv_perm, c->perm, first, n, dout);
+        if (idx == PLB_INDEX_TILE8) PLB_READ_TABLE(PLB_LAYOUT_SOA, PLB_INDEX_TILE8);
+        else if (idx == PLB_INDEX_BITMAP) PLB_READ_TABLE(PLB_LAYOUT_SOA, PLB_INDEX_BITMAP);
+        else PLB_READ_TABLE(PLB_LAYOUT_SOA, PLB_INDEX_I32);
     } else {
-        if (t8)
-            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8>), g, b, 0, c->stream,
-                               a, c->inv_perm, c->perm, first, n, dout);
-        else
-            hipLaunchKernelGGL((k_read_table<PLB_LAYOUT_AOSOA, PLB_INDEX_I32>), g, b, 0, c->stream, a,
-                               c->inv_perm, c->perm, first, n, dout);
+        if (idx == PLB_INDEX_TILE8) PLB_READ_TABLE(PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8);
+        else if (idx == PLB_INDEX_BITMAP) PLB_READ_TABLE(PLB_LAYOUT_AOSOA, PLB_INDEX_BITMAP);
+        else PLB_READ_TABLE(PLB_LAYOUT_AOSOA, PLB_INDEX_I32);
     }
     PLB_HIP(hipGetLastError());
     return PLB_OK;
