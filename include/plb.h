@@ -99,6 +99,10 @@ typedef struct {
     int index_kind;      /* PLB_INDEX_I32 | PLB_INDEX_TILE8 | PLB_INDEX_BITMAP    */
     int nodes_per_thread; /* 1, or 2: a wave advances two 64-node tiles at once
                              (block_size <= 512; single lattices only)          */
+    int graph_steps;     /* 0: one kernel launch per step; G > 0 (even, <= 4096):
+                            runs of G steps replay one captured HIP graph, which
+                            removes the host launch cost on small lattices
+                            (single lattices only)                             */
 } plb_config;
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/

@@ -49,6 +49,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->fp16_scale_log2 = 0;
     cfg->index_kind = PLB_INDEX_I32;
     cfg->nodes_per_thread = 1;
+    cfg->graph_steps = 0;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -100,6 +101,7 @@ int plb::check_cfg(const plb_config* cfg)
         return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread != 1 && cfg->nodes_per_thread != 2) return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread == 2 && cfg->block_size > 512) return PLB_ERR_INVALID;
+    if (cfg->graph_steps < 0 || cfg->graph_steps > 4096 || cfg->graph_steps % 2) return PLB_ERR_INVALID;
     return PLB_OK;
 }
 
@@ -148,6 +150,7 @@ int plb::ctx_alloc_state(plb_ctx* c)
     (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
     c->red_blocks = std::max(1, std::min<int>(dev_cus * 4, (int)((c->N + 255) / 256)));
     PLB_HIP(hipMalloc(&c->red, sizeof(double) * (2 * c->red_blocks + 2)));
+    PLB_HIP(hipMalloc(&c->it_dev, sizeof(int64_t)));
     PLB_HIP(hipStreamSynchronize(c->stream));
     return plb_init_equilibrium(c);
 }
@@ -165,6 +168,9 @@ extern "C" void plb_destroy(plb_ctx* c)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipGraphExec_t& g : c->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (c->it_dev) (void)hipFree(c->it_dev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -275,7 +281,14 @@ extern "C" int plb_step_async(plb_ctx* c, int64_t it0, int64_t nsteps)
         c->pending_it0 = it0;
         c->cur_before = c->cur;
     }
-    for (int64_t s = 0; s < nsteps; s++) {
+    int64_t s = 0;
+    // Runs of graph_steps steps replay a captured graph; the last step of the
+    // call is always a plain launch (it writes the macros).
+    for (const int64_t G = c->cfg.graph_steps; G > 0 && nsteps - s > G; s += G) {
+        int rc = launch_graph_steps(c, it0 + s, G);
+        if (rc) return rc;
+    }
+    for (; s < nsteps; s++) {
         int rc = launch_step(c, it0 + s, s == nsteps - 1);
         if (rc) return rc;
     }

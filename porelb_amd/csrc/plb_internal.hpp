@@ -117,6 +117,8 @@ struct plb_ctx {
     uint32_t* vox = nullptr;    // BITMAP index: node -> voxel position in node order
     int py = 0;                 // BITMAP index: rows per pencil (ny for the V order)
     int64_t n_bm_words = 0;
+    int64_t* it_dev = nullptr;  // first step number of a graph replay
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // G-step graphs by starting buffer
     double index_bytes = 72.0;  // pull-table bytes per node
     uint32_t* perm = nullptr;   // internal -> V order (null: identity)
     uint32_t* inv_perm = nullptr;
@@ -172,7 +174,8 @@ struct DevBuf {
 namespace plb {
 int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n);
 int build_table_from_flag(plb_ctx* c, const uint8_t* flag);
-int launch_step(plb_ctx* c, int64_t it, bool write_macros);
+int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_base = nullptr);
+int launch_graph_steps(plb_ctx* c, int64_t it0, int64_t G);
 int launch_reduce(plb_ctx* c, double* host_out2);
 int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, hipStream_t s);
 int finalize_table(plb_ctx* c, int32_t* tabref, const uint64_t* xyz_ref);

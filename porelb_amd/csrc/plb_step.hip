@@ -46,7 +46,16 @@ struct StepArgs {
     const uint32_t* vox;
     uint32_t nx, ny, nz, py;
     FastDiv div_nx, div_py, div_nz;
+    // Graph replays: the step number is it + *it_base (set before each replay).
+    const int64_t* it_base;
 };
+
+__device__ __forceinline__ StepArgs with_step(const StepArgs& a)
+{
+    StepArgs b = a;
+    if (a.it_base) b.it += *a.it_base;
+    return b;
+}
 
 // ---------------------------------------------------------------- FP64, reference order
 __device__ __forceinline__ double feq_ref(int k, double rho, double u, double v, double w)
@@ -262,7 +271,8 @@ __device__ __forceinline__ void collide_f64(const StepArgs& a, int64_t i, const 
 
 // ---------------------------------------------------------------- shifted FP32 / FP16 storage
 __device__ __forceinline__ float to_dev(float v, float) { return v; }
-__device__ __forceinline__ float to_dev(__half v, float inv) { return __half2float(v) * inv; }
+// fma(convert(h), inv, +0) is the rounded product, in one mixed-precision FMA
+__device__ __forceinline__ float to_dev(__half v, float inv) { return fmaf(__half2float(v), inv, 0.0f); }
 __device__ __forceinline__ void st_dev(float* p, int64_t k, float v, float)
 {
     __builtin_nontemporal_store(v, p + k);
@@ -362,15 +372,16 @@ __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
-    node_any<S, LAYOUT, IDX>(a, i, *a.fault);
+    node_any<S, LAYOUT, IDX>(with_step(a), i, *a.fault);
 }
 
 // Single lattice, two nodes per thread: each wave covers two consecutive
 // 64-node tiles and keeps both tiles' index loads and gathers in flight
 // together (more bytes in flight per wave at the cost of registers).
 template <typename S, int LAYOUT, int IDX>
-__global__ void __launch_bounds__(512) k_step2(const StepArgs a)
+__global__ void __launch_bounds__(512) k_step2(const StepArgs a0)
 {
+    const StepArgs a = with_step(a0);
     const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
     const int64_t t = blk * blockDim.x + threadIdx.x;
     const int64_t i0 = (t >> 6) * 128 + (t & 63);
@@ -694,16 +705,18 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.div_nx = make_fastdiv(a.nx);
     a.div_py = make_fastdiv(a.py);
     a.div_nz = make_fastdiv(a.nz);
+    a.it_base = nullptr;
     return a;
 }
 
-int launch_step(plb_ctx* c, int64_t it, bool write_macros)
+int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_base)
 {
     if (c->N == 0) {  // a mask without pores: nothing to update
         c->cur ^= 1;
         return PLB_OK;
     }
-    const StepArgs a = make_args(c, it, write_macros);
+    StepArgs a = make_args(c, it, write_macros);
+    a.it_base = it_base;
     const int bs = c->cfg.block_size;
     if (c->cfg.nodes_per_thread == 2) {
         const int64_t threads = (c->N + 127) / 128 * 64;
@@ -715,6 +728,41 @@ int launch_step(plb_ctx* c, int64_t it, bool write_macros)
     }
     PLB_HIP(hipGetLastError());
     c->cur ^= 1;
+    return PLB_OK;
+}
+
+__global__ void k_set_step(int64_t* it_dev, int64_t it)
+{
+    if (threadIdx.x == 0) *it_dev = it;
+}
+
+// G steps (G even, so the buffers end where they started) from a graph
+// captured once per starting buffer.  The step number is not baked into the
+// graph: every kernel adds *it_dev, which a plain launch sets before each
+// replay.  The graph holds kernel nodes only.
+int launch_graph_steps(plb_ctx* c, int64_t it0, int64_t G)
+{
+    hipStream_t s = c->stream;
+    const int p = c->cur;
+    if (!c->gexec[p]) {
+        PLB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = PLB_OK;
+        for (int64_t k = 0; k < G && rc == PLB_OK; k++) rc = launch_step(c, k, false, c->it_dev);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        c->cur = p;  // capture does not run anything
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PLB_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PLB_HIP(ei);
+    }
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, s, c->it_dev, it0);
+    PLB_HIP(hipGetLastError());
+    PLB_HIP(hipGraphLaunch(c->gexec[p], s));
     return PLB_OK;
 }
 

@@ -17,7 +17,7 @@
  *   PLB_TAU (0.8)  PLB_GX (1e-3)  PLB_RHO0 (1)  PLB_SAVE (1000)
  *   PLB_PRECISION fp64|fp32|fp16 (fp64)  PLB_BB refcompat|halfway (refcompat)
  *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick|pencil  PLB_BRICK bx,by,bz
- *   PLB_INDEX i32|tile8  PLB_BLOCK (256)
+ *   PLB_INDEX i32|tile8|bitmap  PLB_BLOCK (256)  PLB_GRAPH (0: steps per graph replay)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
  *   PLB_AUTOTUNE cachefile: time layout x node order x block size x nodes per
  *   thread on this V.bin once and reuse the result from cachefile after
@@ -129,8 +129,9 @@ int main(int argc, char *argv[])
             exit(-10);
         }
     }
-    cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", NULL, PLB_INDEX_I32);
+    cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", "bitmap", PLB_INDEX_I32);
     cfg.block_size = env_int("PLB_BLOCK", cfg.block_size);
+    cfg.graph_steps = env_int("PLB_GRAPH", cfg.graph_steps);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
     const int print = env_int("PLB_PRINT", 1);
