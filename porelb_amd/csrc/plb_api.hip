@@ -162,7 +162,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
     }
-    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->twide, c->bm, c->vox, c->perm,
+    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
                     c->inv_perm, c->node_xyz, c->macros, c->fault, c->red};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -284,7 +284,7 @@ extern "C" int plb_step_async(plb_ctx* c, int64_t it0, int64_t nsteps)
     int64_t s = 0;
     // Runs of graph_steps steps replay a captured graph; the last step of the
     // call is always a plain launch (it writes the macros).
-    for (const int64_t G = c->cfg.graph_steps; G > 0 && nsteps - s > G; s += G) {
+    for (const int64_t G = c->N > 0 ? c->cfg.graph_steps : 0; G > 0 && nsteps - s > G; s += G) {
         int rc = launch_graph_steps(c, it0 + s, G);
         if (rc) return rc;
     }

@@ -506,13 +506,15 @@ k_tile8_flags(const int32_t* __restrict__ tab, int64_t Npad, int64_t N, uint64_t
 }
 
 // Offsets are packed four directions per 32-bit word: word k of a lane holds
-// directions 4k+1..4k+4 in bytes 0..3 ([tile][5][64] words, 20 bytes per
-// node), so a lane reads its 18 offsets with 5 loads.
+// directions 4k+1..4k+4 in bytes 0..3 ([tile][4][64] words), and directions
+// 17 and 18 sit in a u16 per lane ([tile][64]): 18 bytes per node, read with
+// four dword loads and one short load.
 template <int LAYOUT>
 __global__ void __launch_bounds__(64)
 k_tile8_write(const int32_t* __restrict__ tab, int64_t Npad, int64_t N,
               const uint64_t* __restrict__ wide, const uint64_t* __restrict__ slot,
-              int32_t* __restrict__ desc, uint32_t* __restrict__ off, int32_t* __restrict__ wtab)
+              int32_t* __restrict__ desc, uint32_t* __restrict__ off, uint16_t* __restrict__ off16,
+              int32_t* __restrict__ wtab)
 {
     const int64_t tile = blockIdx.x;
     const int lane = threadIdx.x;
@@ -538,7 +540,8 @@ k_tile8_write(const int32_t* __restrict__ tab, int64_t Npad, int64_t N,
         words[jm1 >> 2] = (words[jm1 >> 2] & ~(0xffu << sh)) | (o << sh);
     }
 #pragma unroll
-    for (int k = 0; k < TILE8_WORDS; k++) off[(tile * TILE8_WORDS + k) * 64 + lane] = words[k];
+    for (int k = 0; k < TILE8_FULL; k++) off[(tile * TILE8_FULL + k) * 64 + lane] = words[k];
+    off16[tile * 64 + lane] = (uint16_t)(words[TILE8_FULL] & 0xffffu);
 }
 
 // Replaces the int32 table of a context by the TILE8 encoding.
@@ -565,19 +568,20 @@ int compress_tile8(plb_ctx* c)
     PLB_HIP(hipStreamSynchronize(s));
     c->n_wide = (int64_t)(last_slot + last_wide);
     PLB_HIP(hipMalloc(&c->tdesc, sizeof(int32_t) * nk));
-    PLB_HIP(hipMalloc(&c->toff, sizeof(uint32_t) * T * TILE8_WORDS * 64));
+    PLB_HIP(hipMalloc(&c->toff, sizeof(uint32_t) * T * TILE8_FULL * 64));
+    PLB_HIP(hipMalloc(&c->toff16, sizeof(uint16_t) * T * 64));
     PLB_HIP(hipMalloc(&c->twide, sizeof(int32_t) * 64 * (c->n_wide ? c->n_wide : 1)));
     if (c->cfg.layout == PLB_LAYOUT_SOA)
         hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_SOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->twide);
+                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->toff16, c->twide);
     else
         hipLaunchKernelGGL(k_tile8_write<PLB_LAYOUT_AOSOA>, dim3((unsigned)T), dim3(64), 0, s, c->tab,
-                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->twide);
+                           c->Npad, c->N, wide.p, slot.p, c->tdesc, c->toff, c->toff16, c->twide);
     PLB_HIP(hipGetLastError());
     PLB_HIP(hipStreamSynchronize(s));
     (void)hipFree(c->tab);
     c->tab = nullptr;
-    c->index_bytes = c->N ? ((double)T * TILE8_WORDS * 256 + (double)nk * 4 + (double)c->n_wide * 256) /
+    c->index_bytes = c->N ? ((double)T * (TILE8_FULL * 256 + 128) + (double)nk * 4 + (double)c->n_wide * 256) /
                                 (double)c->N
                           : 0.0;
     return PLB_OK;

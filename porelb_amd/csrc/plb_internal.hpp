@@ -39,7 +39,8 @@ constexpr double TP[Q] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 
 constexpr int32_t TAB_BOUNCE = -1;
 constexpr int32_t TAB_CONST = -2;
 constexpr unsigned long long NO_FAULT = ~0ull;
-constexpr int TILE8_WORDS = 5;  // u32 words of packed u8 offsets per node
+constexpr int TILE8_WORDS = 5;  // words of packed u8 offsets per node: 4 u32 + one u16
+constexpr int TILE8_FULL = 4;   // the u32 words; directions 17 and 18 live in the u16
 
 // Position of population (dir j, node n).
 template <int LAYOUT>
@@ -111,6 +112,7 @@ struct plb_ctx {
     int32_t* tab = nullptr;     // internal layout (I32 index)
     int32_t* tdesc = nullptr;   // TILE8 index
     uint32_t* toff = nullptr;
+    uint16_t* toff16 = nullptr;
     int32_t* twide = nullptr;
     int64_t n_wide = 0;
     uint2* bm = nullptr;        // BITMAP index: {pore bits, pores before} per 32 voxels

@@ -32,6 +32,7 @@ struct StepArgs {
     const int32_t* tab;
     const int32_t* tdesc;
     const uint32_t* toff;
+    const uint16_t* toff16;
     const int32_t* twide;
     int64_t N, Npad;
     double omega, gx, rho0;
@@ -148,18 +149,21 @@ __device__ __forceinline__ int32_t bitmap_lookup(uint2 wd, uint32_t v)
 template <int LAYOUT, int IDX>
 __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
 {
+    // Index words are read once per step: non-temporal loads keep them from
+    // displacing population lines that neighbouring tiles read again.
     if constexpr (IDX == PLB_INDEX_I32) {
 #pragma unroll
-        for (int j = 1; j < Q; j++) e.t[j] = a.tab[tpos<LAYOUT>(j - 1, i, a.Npad)];
+        for (int j = 1; j < Q; j++) e.t[j] = __builtin_nontemporal_load(a.tab + tpos<LAYOUT>(j - 1, i, a.Npad));
     } else if constexpr (IDX == PLB_INDEX_BITMAP) {
-        e.v = a.vox[i];
+        e.v = __builtin_nontemporal_load(a.vox + i);
     } else {
         // TILE8: the wave is one 64-node tile, so its 18 descriptors are
         // uniform (scalar loads); each lane reads 5 words of packed offsets.
         const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
-        const uint32_t* o = a.toff + tile * TILE8_WORDS * 64 + (i & 63);
+        const uint32_t* o = a.toff + tile * TILE8_FULL * 64 + (i & 63);
 #pragma unroll
-        for (int k = 0; k < TILE8_WORDS; k++) e.w[k] = o[k * 64];
+        for (int k = 0; k < TILE8_FULL; k++) e.w[k] = __builtin_nontemporal_load(o + k * 64);
+        e.w[TILE8_FULL] = __builtin_nontemporal_load(a.toff16 + tile * 64 + (i & 63));
         const int32_t* d = a.tdesc + tile * (Q - 1);
 #pragma unroll
         for (int j = 1; j < Q; j++) e.desc[j] = d[j - 1];
@@ -526,7 +530,9 @@ __global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const u
         for (int j = 1; j < Q; j++) {
             const int32_t desc = a.tdesc[tile * (Q - 1) + j - 1];
             if (desc >= 0) {
-                const uint32_t word = a.toff[(tile * TILE8_WORDS + ((j - 1) >> 2)) * 64 + lane];
+                const int k = (j - 1) >> 2;
+                const uint32_t word = k < TILE8_FULL ? a.toff[(tile * TILE8_FULL + k) * 64 + lane]
+                                                     : (uint32_t)a.toff16[tile * 64 + lane];
                 const int32_t v = (int32_t)((word >> (8 * ((j - 1) & 3))) & 0xffu);
                 t[j] = v == 255 ? TAB_BOUNCE : (v == 254 ? TAB_CONST : desc + v);
             } else {
@@ -683,6 +689,7 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.tab = c->tab;
     a.tdesc = c->tdesc;
     a.toff = c->toff;
+    a.toff16 = c->toff16;
     a.twide = c->twide;
     a.N = c->N;
     a.Npad = c->Npad;

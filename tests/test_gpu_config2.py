@@ -37,3 +37,29 @@ def test_config2_gpu_equals_reference_mpirun_16(plb, tmp_path):
     assert sha(f) == sha(ref["f0"])
     assert sha(mac) == sha(ref["macros"])
     assert ref["timing"]["ranks"] == ranks
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(index_kind=1, layout=1, ordering=2, brick=(256, 64, 1))])
+def test_config2_20000_steps_matches_reference_digest(plb, kw):
+    """Config 2 as specified: 20 000 steps on the 16-rank V.bin.  The
+    reference's final populations and part files (tests/golden/config2_20000.json,
+    written by make_config2_golden.py from `mpirun -np 16` of the reference
+    build) must be reproduced bit for bit; the V.bin comes from the GPU
+    preprocess."""
+    path = os.path.join(REPO, "tests", "golden", "config2_20000.json")
+    if not os.path.exists(path):
+        pytest.skip("config2_20000.json not generated")
+    import json
+
+    g = json.load(open(path))
+    m = g["mask"]
+    flag = plb.mask_spheres(tuple(m["dims"]), m["rmin"], m["rmax"], m["porosity"], m["seed"])
+    assert sha(flag) == m["sha256"]
+    V, start = plb.preprocess(flag, g["ranks"])
+    print("[case] config2 20000", kw, len(V), flush=True)
+    with plb.Lattice.from_V(V, start, flag.shape, tau=g["tau"], gx=g["gx"], **kw) as lat:
+        lat.step(0, g["steps"])
+        f = lat.read_populations()
+        mac = lat.read_macros()
+    assert sha(f) == g["f0_sha256"]
+    assert sha(mac) == g["macros_sha256"]

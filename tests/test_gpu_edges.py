@@ -19,8 +19,9 @@ def oracle_run(flag, np_, steps, tau, gx, mode=O.BB_REFCOMPAT):
 
 def test_no_pores(plb):
     flag = np.zeros((4, 5, 6), dtype=np.uint8)
-    for index in (0, 1):
-        with plb.Lattice.from_flag(flag, index_kind=index) as lat:
+    for kw in (dict(index_kind=0), dict(index_kind=1), dict(index_kind=2, bb_mode=1),
+               dict(index_kind=1, graph_steps=2)):
+        with plb.Lattice.from_flag(flag, **kw) as lat:
             assert lat.N == 0
             lat.step(0, 5)
             assert lat.read_populations().shape == (0, 19)
@@ -37,7 +38,10 @@ def test_degenerate_dims_bitwise(plb, dims, mode):
     flag = (rng.random(dims) < 0.7).astype(np.uint8)
     flag.flat[0] = 1
     V, start, f0, mac = oracle_run(flag, 1, 20, 0.8, 1e-4, mode)
-    for kw in (dict(), dict(index_kind=1, layout=1), dict(ordering=2, brick=(dims[2], 1, 1))):
+    variants = [dict(), dict(index_kind=1, layout=1), dict(ordering=2, brick=(dims[2], 1, 1))]
+    if mode == O.BB_HALFWAY:
+        variants += [dict(index_kind=2), dict(index_kind=2, layout=1, ordering=2, brick=(dims[2], 1, 1))]
+    for kw in variants:
         with plb.Lattice.from_flag(flag, bb_mode=mode, tau=0.8, gx=1e-4, **kw) as lat:
             assert np.array_equal(lat.read_table(), O.pull_table(V, start, mode))
             lat.step(0, 20)
