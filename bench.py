@@ -189,6 +189,45 @@ def study_precision(P, max_steps, check, tol):
     print(json.dumps(out), flush=True)
 
 
+def study_config4(P, steps, chunk=500):
+    """Config 4: the 1024^3 sphere pack (porosity 0.20, tau 0.6, g_x 5e-7),
+    the same number of steps in FP64 and in FP32 storage (both half-way
+    bounce-back, TILE8, pencils).  Reports MLUPS and the roofline fraction of
+    each run, and the FP32 errors in K and in the velocity field against FP64."""
+    w = WORKLOADS["spheres1024-fp32"]
+    flag = make_mask(P, w)
+    runs = {}
+    for prec in (0, 1):
+        cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
+                               ordering=2, brick=w["brick"], block_size=256)
+        with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
+            m0 = lat.reduce()["mass"]
+            ms, it = 0.0, 0
+            while it < steps:
+                n = min(chunk, steps - it)
+                ms += lat.step_timed(it, n)
+                it += n
+                log(f"config4 precision {prec}: step {it}")
+            r = lat.reduce()
+            mac = lat.read_macros()
+            mlups = lat.N * steps / (ms * 1e-3) / 1e6
+            runs[prec] = dict(K=r["K"], mean_ux=r["mean_ux"], mass_drift=(r["mass"] - m0) / m0, mlups=mlups,
+                              roofline_frac=mlups * 1e6 * (2 * 19 * STORAGE_BYTES[prec] + 19 * 4) / (HBM_PEAK_GBS * 1e9),
+                              u=mac[1:].copy(), nodes=lat.N)
+        log(f"config4 precision {prec}: {mlups:.0f} MLUPS, K {r['K']:.9e}")
+    u64 = runs[0]["u"]
+    out = {"study": "config4", "workload": w["label"], "pore_nodes": runs[0]["nodes"], "steps": steps,
+           "bounce_back": "halfway", "results": {}}
+    for prec, name in ((0, "fp64"), (1, "fp32_store")):
+        r = runs[prec]
+        out["results"][name] = {
+            "mlups": round(r["mlups"], 1), "roofline_frac": round(r["roofline_frac"], 4), "K": r["K"],
+            "mean_ux": r["mean_ux"], "mass_rel_drift": r["mass_drift"],
+            "velocity_rel_l2_err": float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64)),
+            "K_rel_err": abs(r["K"] / runs[0]["K"] - 1.0)}
+    print(json.dumps(out), flush=True)
+
+
 def study_batch(P, n_samples, max_steps, check, tol, seed=100):
     """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
     sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
@@ -243,7 +282,7 @@ def main():
     ap.add_argument("--ordering", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
-    ap.add_argument("--study", choices=["precision", "batch"], default=None,
+    ap.add_argument("--study", choices=["precision", "batch", "config4"], default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--check", type=int, default=200)
@@ -262,6 +301,9 @@ def main():
         return
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6)
+        return
+    if args.study == "config4":
+        study_config4(P, args.max_steps if args.max_steps != 100000 else 5000)
         return
     w = WORKLOADS[args.workload]
     cpu = None

@@ -89,7 +89,7 @@ __device__ __forceinline__ bool frozen(unsigned long long fk, int64_t it)
 // ---------------------------------------------------------------- phases of one node
 // A node update runs in four phases so a thread can interleave two nodes
 // (issue both nodes' index loads, then both nodes' gathers, before waiting):
-//   issue_entries  -> index loads (I32: 18 words; TILE8: 5 words + 18 scalar
+//   issue_entries  -> index loads (I32: 18 words; TILE8: 4 words + 1 short + 18 scalar
 //                     descriptors of the wave's tile)
 //   decode_entries -> t[1..18]: >= 0 source node, -1 bounce, -2 constant
 //   gather         -> the 19 stored populations
@@ -158,7 +158,8 @@ __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entr
         e.v = __builtin_nontemporal_load(a.vox + i);
     } else {
         // TILE8: the wave is one 64-node tile, so its 18 descriptors are
-        // uniform (scalar loads); each lane reads 5 words of packed offsets.
+        // uniform (scalar loads); each lane reads 4 words and a short of packed
+        // offsets.
         const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
         const uint32_t* o = a.toff + tile * TILE8_FULL * 64 + (i & 63);
 #pragma unroll
