@@ -290,7 +290,7 @@ def main():
     ap.add_argument("--samples", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is not supported "
               "(the domain of every configuration fits one MI355X)", file=sys.stderr)
         sys.exit(2)
 

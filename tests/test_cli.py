@@ -114,3 +114,12 @@ def test_postprocess_permeability_report(tmp_path, plb):
     q = s_ux / flag.size
     assert float(got["q"]) == q
     assert float(got["K"]) == 1.0 / 3.0 * (0.9 - 0.5) * q / 2e-5
+
+
+def test_bench_refuses_several_gpus(tmp_path):
+    """The step runs on one GPU; asking bench.py for more exits before any GPU work."""
+    import sys
+
+    r = run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], tmp_path)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

@@ -151,3 +151,26 @@ def test_postprocess_K_matches_device_reduction(tmp_path, plb):
     for k in ("sum_ux", "mean_ux", "q", "K"):
         assert got[k] == pytest.approx(red[k], rel=1e-12), k
     assert got["K"] > 0
+
+
+def test_porelb_fp32_pencil_tile8_close_to_fp64(tmp_path):
+    """The chosen forms through the drop-in CLI: FP32 storage, half-way
+    bounce-back, TILE8, AoSoA and pencils write part files within FP32 error
+    of the FP64 half-way run."""
+    flag = (np.random.default_rng(21).random((20, 18, 24)) < 0.62).astype(np.uint8)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    base = {"PLB_TAU": "0.9", "PLB_GX": "2e-5", "PLB_PRINT": "0", "PLB_BB": "halfway"}
+    out = {}
+    for name, env in [("fp64", {}), ("fp32", {"PLB_PRECISION": "fp32", "PLB_INDEX": "tile8", "PLB_LAYOUT": "aosoa",
+                                              "PLB_ORDER": "pencil", "PLB_BRICK": "24,4,1", "PLB_GRAPH": "8"})]:
+        wd = tmp_path / name
+        wd.mkdir()
+        os.link(tmp_path / "V.bin", wd / "V.bin")
+        r = run([os.path.join(BIN, "porelb.x"), "400"], wd, dict(base, **env))
+        assert r.returncode == 0, r.stderr
+        out[name] = read_parts(wd, 2, 400)
+    ux64, ux32 = out["fp64"][1], out["fp32"][1]
+    assert np.abs(out["fp32"][0] - out["fp64"][0]).max() < 1e-6
+    assert np.linalg.norm(ux32 - ux64) / np.linalg.norm(ux64) < 1e-4
