@@ -26,14 +26,16 @@ typedef struct {
     double ms;      /* device time of `steps` steps */
 } plb_tune_result;
 
-/* Tries layout (SOA, AOSOA) x node order (reference; 8^3 bricks; pencils of
- * 64 and 128 full rows) x block size (128, 256) x nodes per thread (1, 2)
- * for `base` (precision, bounce-back, index encoding, tau, gx kept) on the
- * mask, `steps` timed steps each, and writes the fastest configuration to
- * *best.  When cache_path is not NULL, a result stored there for the same
- * mask (64-bit hash of the voxels and dimensions), precision, bounce-back
- * mode and index encoding is returned without timing, and new results are
- * appended (one text line per result).
+/* Times candidate launch shapes for `base` (precision, bounce-back, tau, gx
+ * kept) on the mask, `steps` timed steps each, and writes the fastest
+ * configuration to *best.  Stage 1: index encoding (I32, TILE8, and BITMAP
+ * for half-way bounce-back) x layout (SOA, AOSOA) x node order (reference;
+ * 8^3 bricks; pencils of 64 and 128 full rows) at 256 threads.  Stage 2:
+ * block size (128, 256, 512) x nodes per thread (1, 2) for the stage-1
+ * winner.  All index encodings give bit-identical results.  When cache_path
+ * is not NULL, a result stored there for the same mask (64-bit hash of the
+ * voxels and dimensions), precision and bounce-back mode is returned without
+ * timing, and new results are appended (one text line per result).
  * results (may be NULL) receives up to max_results timed candidates. */
 int plb_autotune(const plb_config *base, const uint8_t *flag, int nx, int ny, int nz, int num_p,
                  int steps, const char *cache_path, plb_config *best, plb_tune_result *results,

@@ -305,7 +305,9 @@ def copy_bandwidth(nbytes=8 << 30, iters=5, device=0):
 
 
 def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
-    """Fastest (layout, order, block) for this mask; returns (Config, results)."""
+    """Fastest (index encoding, layout, node order, block size, nodes per
+    thread) for this mask, by plb_autotune's two-stage search; returns
+    (Config, results)."""
     cfg = cfg or default_config(**kw)
     flag = np.ascontiguousarray(flag, dtype=np.uint8)
     nz, ny, nx = flag.shape
@@ -318,8 +320,8 @@ def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
     out = []
     for r in res[:n.value]:
         c = r.cfg
-        out.append(dict(layout=c.layout, ordering=c.ordering, brick=tuple(c.brick), block=c.block_size,
-                        npt=c.nodes_per_thread, mlups=r.mlups, ms=r.ms))
+        out.append(dict(index_kind=c.index_kind, layout=c.layout, ordering=c.ordering, brick=tuple(c.brick),
+                        block=c.block_size, npt=c.nodes_per_thread, mlups=r.mlups, ms=r.ms))
     return best, out
 
 

@@ -89,7 +89,8 @@ static bool cache_lookup(const char* path, uint64_t key, int prec, int bb, plb_c
         int p, b, layout, order, bx, by, bz, bs, npt, idx;
         if (sscanf(line, "%llx %d %d %d %d %d %d %d %d %d %d", &k, &p, &b, &idx, &layout, &order, &bx,
                    &by, &bz, &bs, &npt) == 11 &&
-            k == key && p == prec && b == bb && idx == best->index_kind) {
+            k == key && p == prec && b == bb) {
+            best->index_kind = idx;
             best->layout = layout;
             best->ordering = order;
             best->brick[0] = bx;
@@ -118,55 +119,74 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
     if (cache_path && cache_lookup(cache_path, key, base->precision, base->bb_mode, best))
         return PLB_OK;
 
-    struct Cand {
-        int layout, order, bx, by, bz, bs, npt;
+    // Stage 1: index encoding x layout x node order at 256 threads, one node
+    // per thread.  Stage 2: block size and nodes per thread for the winner.
+    // Every index encoding gives the same sources, so the choice changes speed
+    // only, never results.
+    double best_mlups = -1.0;
+    int nres = 0;
+    auto time_cfg = [&](const plb_config& cfg, double* mlups) -> int {
+        plb_ctx* c = nullptr;
+        int r = plb_create_from_flag(&c, &cfg, flag, nx, ny, nz, num_p);
+        if (r) return r;
+        double ms = 0.0;
+        r = plb_step(c, 0, std::max(2, steps / 4));
+        if (!r) r = plb_step_timed(c, 0, steps, &ms);
+        const double n = (double)plb_num_nodes(c);
+        plb_destroy(c);
+        if (r) return r;
+        *mlups = n * steps / (ms * 1e-3) / 1e6;
+        if (results && nres < max_results) results[nres] = {cfg, *mlups, ms};
+        nres++;
+        if (*mlups > best_mlups) {
+            best_mlups = *mlups;
+            *best = cfg;
+        }
+        return PLB_OK;
     };
-    std::vector<Cand> cands;
     // node orders: reference; 8^3 bricks; pencils of 64 and 128 full rows
     const int orders[][4] = {{PLB_ORDER_REFERENCE, 0, 0, 0},
                              {PLB_ORDER_BRICK, 8, 8, 8},
                              {PLB_ORDER_PENCIL, nx, 64, 1},
                              {PLB_ORDER_PENCIL, nx, 128, 1}};
-    for (int layout = 0; layout < 2; layout++)
-        for (const auto& o : orders)
-            for (int bs : {128, 256})
-                for (int npt : {1, 2})
-                    cands.push_back({layout, o[0], o[0] ? o[1] : base->brick[0],
-                                     o[0] ? o[2] : base->brick[1], o[0] ? o[3] : base->brick[2], bs, npt});
-    double best_mlups = -1.0;
-    int nres = 0;
-    for (const Cand& cd : cands) {
-        plb_config cfg = *base;
-        cfg.layout = cd.layout;
-        cfg.ordering = cd.order;
-        cfg.brick[0] = cd.bx;
-        cfg.brick[1] = cd.by;
-        cfg.brick[2] = cd.bz;
-        cfg.block_size = cd.bs;
-        cfg.nodes_per_thread = cd.npt;
-        plb_ctx* c = nullptr;
-        rc = plb_create_from_flag(&c, &cfg, flag, nx, ny, nz, num_p);
-        if (rc) return rc;
-        double ms = 0.0;
-        rc = plb_step(c, 0, std::max(2, steps / 4));
-        if (!rc) rc = plb_step_timed(c, 0, steps, &ms);
-        const double n = (double)plb_num_nodes(c);
-        plb_destroy(c);
-        if (rc) return rc;
-        const double mlups = n * steps / (ms * 1e-3) / 1e6;
-        if (results && nres < max_results) results[nres] = {cfg, mlups, ms};
-        nres++;
-        if (mlups > best_mlups) {
-            best_mlups = mlups;
-            *best = cfg;
+    std::vector<int> kinds = {PLB_INDEX_I32, PLB_INDEX_TILE8};
+    if (base->bb_mode == PLB_BB_HALFWAY) kinds.push_back(PLB_INDEX_BITMAP);
+    for (int idx : kinds)
+        for (int layout = 0; layout < 2; layout++)
+            for (const auto& o : orders) {
+                if (idx == PLB_INDEX_BITMAP && o[0] == PLB_ORDER_BRICK) continue;
+                plb_config cfg = *base;
+                cfg.index_kind = idx;
+                cfg.layout = layout;
+                cfg.ordering = o[0];
+                if (o[0] != PLB_ORDER_REFERENCE) {
+                    cfg.brick[0] = o[1];
+                    cfg.brick[1] = o[2];
+                    cfg.brick[2] = o[3];
+                }
+                cfg.block_size = 256;
+                cfg.nodes_per_thread = 1;
+                cfg.graph_steps = 0;
+                double m = 0.0;
+                rc = time_cfg(cfg, &m);
+                if (rc) return rc;
+            }
+    const plb_config stage1 = *best;
+    for (int bs : {128, 256, 512})
+        for (int npt : {1, 2}) {
+            plb_config cfg = stage1;
+            cfg.block_size = bs;
+            cfg.nodes_per_thread = npt;
+            double m = 0.0;
+            rc = time_cfg(cfg, &m);
+            if (rc) return rc;
         }
-    }
     if (n_results) *n_results = std::min(nres, max_results);
     if (cache_path) {
         FILE* fp = fopen(cache_path, "a");
         if (fp) {
             fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
-                    base->precision, base->bb_mode, base->index_kind, best->layout, best->ordering,
+                    base->precision, base->bb_mode, best->index_kind, best->layout, best->ordering,
                     best->brick[0], best->brick[1], best->brick[2], best->block_size,
                     best->nodes_per_thread, best_mlups);
             fclose(fp);

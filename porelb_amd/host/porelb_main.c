@@ -19,8 +19,9 @@
  *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick|pencil  PLB_BRICK bx,by,bz
  *   PLB_INDEX i32|tile8|bitmap  PLB_BLOCK (256)  PLB_GRAPH (0: steps per graph replay)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
- *   PLB_AUTOTUNE cachefile: time layout x node order x block size x nodes per
- *   thread on this V.bin once and reuse the result from cachefile after
+ *   PLB_AUTOTUNE cachefile: time index encoding x layout x node order, then
+ *   block size x nodes per thread, on this V.bin once; reuse the result from
+ *   cachefile after
  *   PLB_CHECKPOINT file: populations written after the last step
  *   PLB_RESTART file: start from such a checkpoint (fills the empty
  *   save_state/recovery_state stubs of porelb.c:367-373)
@@ -162,9 +163,9 @@ int main(int argc, char *argv[])
             exit(rc);
         }
         cfg = best;
-        printf(">>> autotune: layout %d order %d brick %d,%d,%d block %d nodes/thread %d\n", cfg.layout,
-               cfg.ordering, cfg.brick[0], cfg.brick[1], cfg.brick[2], cfg.block_size,
-               cfg.nodes_per_thread);
+        printf(">>> autotune: index %d layout %d order %d brick %d,%d,%d block %d nodes/thread %d\n",
+               cfg.index_kind, cfg.layout, cfg.ordering, cfg.brick[0], cfg.brick[1], cfg.brick[2],
+               cfg.block_size, cfg.nodes_per_thread);
     }
 
     plb_ctx *c = NULL;
