@@ -123,3 +123,23 @@ def test_bench_refuses_several_gpus(tmp_path):
     r = run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], tmp_path)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_postprocess_errors(tmp_path, plb):
+    """postprocess.c:92-112: bad mode, missing V.bin, slice out of range."""
+    r = run([os.path.join(BIN, "postprocess.x")], tmp_path)
+    assert r.returncode == (-2) & 0xFF and "Usage" in r.stdout
+    r = run([os.path.join(BIN, "postprocess.x"), "Q", "1"], tmp_path)
+    assert r.returncode == (-3) & 0xFF and "Not implemented yet!" in r.stderr
+    r = run([os.path.join(BIN, "postprocess.x"), "K", "10"], tmp_path)
+    assert r.returncode == (-1) & 0xFF and "Node map file openning error!" in r.stderr
+    import oracle as O
+
+    flag = np.ones((3, 4, 5), dtype=np.uint8)
+    V, start = O.build_V(flag, 1)
+    plb.write_vbin(str(tmp_path / "V.bin"), V, start, flag.shape)
+    flag.tofile(tmp_path / "flag.bin")
+    r = run([os.path.join(BIN, "postprocess.x"), "X", "5", "10"], tmp_path)
+    assert r.returncode == (-3) & 0xFF and "out of range" in r.stderr
+    r = run([os.path.join(BIN, "postprocess.x"), "K", "10"], tmp_path)  # part file missing
+    assert r.returncode == (-1) & 0xFF and "Part file openning error!" in r.stderr
