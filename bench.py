@@ -228,6 +228,63 @@ def study_config4(P, steps, chunk=500):
     print(json.dumps(out), flush=True)
 
 
+def study_preprocess(P):
+    """The index build (preprocess.c:29-193 and the pull table): the
+    reference's own build_V and save_V (one host core; 256^3 only, larger
+    masks take many minutes) against the GPU build -- plb_preprocess with the
+    V records copied to host memory, preprocess.x end to end including the
+    V.bin write, and plb_create_from_flag for the benchmark's step shape."""
+    import oracle as O
+
+    cli = os.path.join(REPO, "porelb_amd", "bin", "preprocess.x")
+    P.preprocess(np.ones((4, 4, 4), dtype=np.uint8), 1)  # runtime start-up outside the timings
+    out = {"study": "preprocess", "results": []}
+    for key, num_p, ref in (("spheres256-fp64", 16, True), ("noise512-fp32", 1, False),
+                            ("spheres1024-fp32", 1, False)):
+        w = WORKLOADS[key]
+        flag = make_mask(P, w)
+        nz, ny, nx = flag.shape
+        res = {"mask": w["label"].split(",")[0] + f", porosity {w['porosity']}", "num_p": num_p}
+        t0 = time.perf_counter()
+        V, start = P.preprocess(flag, num_p)
+        res["gpu_build_V_to_host_s"] = round(time.perf_counter() - t0, 3)
+        res["pore_nodes"] = int(len(V))
+        del V
+        wd = tempfile.mkdtemp(prefix="plbpre_")
+        try:
+            flag.tofile(os.path.join(wd, "flag.bin"))
+            vfile = "V.bin" if nx * ny * nz <= 512 ** 3 else "/dev/null"
+            t0 = time.perf_counter()
+            subprocess.run([cli, str(num_p), str(nx), str(ny), str(nz), "flag.bin", vfile], cwd=wd, check=True,
+                           capture_output=True, timeout=900)
+            res["gpu_preprocess_x_s"] = round(time.perf_counter() - t0, 3)
+            res["gpu_preprocess_x_writes"] = vfile
+            if ref and os.path.exists(O.REF_PREPROCESS):
+                log(f"reference preprocess on {key}")
+                r = subprocess.run([O.REF_PREPROCESS, str(num_p), str(nx), str(ny), str(nz), "flag.bin", "Vref.bin"],
+                                   cwd=wd, check=True, capture_output=True, text=True, timeout=1800)
+                line = [ln for ln in r.stdout.splitlines() if ln.startswith("PLB_REF_PREPROCESS")][0]
+                t = {k: float(v) for k, v in (tok.split("=") for tok in line.split()[1:])}
+                res["ref_build_V_s"], res["ref_save_V_s"] = t["build_s"], t["save_s"]
+                # SOLID entries and padding are uninitialised in the reference's
+                # records, so the files are compared field by field.
+                a, b = O.read_vbin(os.path.join(wd, "V.bin")), O.read_vbin(os.path.join(wd, "Vref.bin"))
+                res["V_semantic_fields_equal"] = bool(
+                    np.array_equal(a["start"], b["start"]) and
+                    all(np.array_equal(x, y) for x, y in zip(O.semantic_fields(a["V"]), O.semantic_fields(b["V"]))))
+        finally:
+            subprocess.run(["rm", "-rf", wd])
+        sw = WORKLOADS["spheres1024-fp32"]
+        t0 = time.perf_counter()
+        lat = P.Lattice.from_flag(flag, 1, tau=w["tau"], gx=w["gx"], precision=1, bb_mode=1, index_kind=1, layout=1,
+                                  ordering=2, brick=(nx,) + tuple(sw["brick"][1:]), block_size=256)
+        res["gpu_lattice_create_s"] = round(time.perf_counter() - t0, 3)
+        lat.close()
+        log(json.dumps(res))
+        out["results"].append(res)
+    print(json.dumps(out), flush=True)
+
+
 def study_batch(P, n_samples, max_steps, check, tol, seed=100):
     """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
     sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
@@ -282,7 +339,7 @@ def main():
     ap.add_argument("--ordering", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
-    ap.add_argument("--study", choices=["precision", "batch", "config4"], default=None,
+    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess"], default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--check", type=int, default=200)
@@ -301,6 +358,9 @@ def main():
         return
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6)
+        return
+    if args.study == "preprocess":
+        study_preprocess(P)
         return
     if args.study == "config4":
         study_config4(P, args.max_steps if args.max_steps != 100000 else 5000)
