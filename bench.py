@@ -285,6 +285,37 @@ def study_preprocess(P):
     print(json.dumps(out), flush=True)
 
 
+def study_outputs(P, reps=5):
+    """The output side of a step at 1024^3 (FP32 bench shape): the FP64
+    reduction (plb_reduce: populations and u_x read once, 19*4 + 8 B per node)
+    and the macro read-back that save_data needs (plb_read_macros: 4 x 8 B per
+    node gathered to V order and copied to the host)."""
+    w = WORKLOADS["spheres1024-fp32"]
+    flag = make_mask(P, w)
+    with P.Lattice.from_flag(flag, 1, tau=w["tau"], gx=w["gx"], precision=1, bb_mode=1, index_kind=1, layout=1,
+                             ordering=2, brick=w["brick"], block_size=256) as lat:
+        lat.step(0, 10)
+        n = lat.N
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            lat.reduce()
+            t.append(time.perf_counter() - t0)
+        red = min(t)
+        t = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            lat.read_macros()
+            t.append(time.perf_counter() - t0)
+        mac = min(t)
+    out = {"study": "outputs", "workload": w["label"], "pore_nodes": n,
+           "reduce_s": round(red, 5), "reduce_bytes": n * (19 * 4 + 8),
+           "reduce_gbs": round(n * (19 * 4 + 8) / red / 1e9, 1),
+           "reduce_roofline_frac": round(n * (19 * 4 + 8) / red / (HBM_PEAK_GBS * 1e9), 3),
+           "read_macros_s": round(mac, 4), "read_macros_host_gbs": round(n * 32 / mac / 1e9, 2)}
+    print(json.dumps(out), flush=True)
+
+
 def study_batch(P, n_samples, max_steps, check, tol, seed=100):
     """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
     sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
@@ -339,7 +370,7 @@ def main():
     ap.add_argument("--ordering", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
-    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess"], default=None,
+    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess", "outputs"], default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--check", type=int, default=200)
@@ -358,6 +389,9 @@ def main():
         return
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6)
+        return
+    if args.study == "outputs":
+        study_outputs(P)
         return
     if args.study == "preprocess":
         study_preprocess(P)
