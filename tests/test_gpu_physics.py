@@ -1,0 +1,62 @@
+"""Physics known answers through the HIP path.
+
+- Slit Poiseuille flow with half-way bounce-back: exact (to convergence) at
+  tau = 1/2 + sqrt(3/16) for BGK with half-way walls (He, Zou, Luo & Dembo,
+  J. Stat. Phys. 87, 1997), for every index encoding, layout and order, and
+  within FP32 rounding for the shifted FP32 storage.
+- Permeability of a body-centred cubic sphere array against the
+  Sangani-Acrivos series that getK.c tabulates (getK.c:5-72): a few percent at
+  32 voxels per cell."""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_oracle import poiseuille_flag, poiseuille_profile
+
+pytestmark = pytest.mark.gpu
+MAGIC_TAU = 0.5 + np.sqrt(3.0 / 16.0)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.mark.parametrize("kw,tol", [
+    (dict(), 1e-9),
+    (dict(index_kind=1, layout=1, ordering=2, brick=(4, 8, 1)), 1e-9),
+    (dict(index_kind=2, layout=1, ordering=2, brick=(4, 8, 1)), 1e-9),
+    (dict(index_kind=1, layout=1, graph_steps=64), 1e-9),
+    (dict(precision=1, index_kind=1, layout=1, ordering=2, brick=(4, 8, 1)), 1e-4),
+])
+def test_poiseuille_exact_at_magic_tau(plb, kw, tol):
+    ny, g = 20, 1e-6
+    flag = poiseuille_flag(ny)
+    say("poiseuille", kw)
+    with plb.Lattice.from_flag(flag, 1, tau=MAGIC_TAU, gx=g, bb_mode=plb.BB_HALFWAY, **kw) as lat:
+        lat.step(0, 30000)
+        mac = lat.read_macros()  # V order
+    V, _ = O.build_V(flag, 1)
+    ua = poiseuille_profile(V["y"].astype(float), ny, MAGIC_TAU, g)
+    err = np.linalg.norm(mac[1] - ua) / np.linalg.norm(ua)
+    say("poiseuille rel L2 error", err)
+    assert err < tol
+
+
+@pytest.mark.parametrize("chi", [0.5, 0.8])
+def test_bcc_permeability_vs_sangani_acrivos(plb, chi):
+    """K = nu q / g_x of a BCC array (two spheres per cubic cell of 32 voxels,
+    radius sqrt(3) L chi / 4) against K1 of getK.c; FP32 storage agrees with
+    FP64 far inside that margin."""
+    n, tau, g = 32, 1.0, 1e-6
+    flag = plb.mask_bcc(n, chi)
+    k1 = O.getK_bcc(chi, n + 1)["K1"]  # getK.c takes the resolution nx and uses L = nx - 1
+    Ks = {}
+    for prec in (0, 1):
+        say("bcc", chi, prec)
+        with plb.Lattice.from_flag(flag, 1, tau=tau, gx=g, precision=prec, bb_mode=plb.BB_HALFWAY,
+                                   index_kind=1, layout=1, ordering=2, brick=(n, 8, 1)) as lat:
+            lat.step(0, 8000)
+            Ks[prec] = lat.reduce()["K"]
+    say("bcc K", Ks, "K1", k1)
+    assert abs(Ks[0] / k1 - 1.0) < 0.05
+    assert abs(Ks[1] / Ks[0] - 1.0) < 1e-4

@@ -227,3 +227,35 @@ def test_reduce_definition(oracle_mod):
     assert r["sum_ux"] == pytest.approx(mac[1].sum(), rel=1e-14)
     assert r["q"] == pytest.approx(mac[1].sum() / 216, rel=1e-14)
     assert r["K"] == pytest.approx((0.8 - 0.5) / 3 * r["q"] / 1e-4, rel=1e-14)
+
+
+def poiseuille_flag(ny, nx=4, nz=3):
+    """Slit between two solid planes y = 0 and y = ny - 1, periodic in x and z."""
+    flag = np.ones((nz, ny, nx), dtype=np.uint8)
+    flag[:, 0, :] = 0
+    flag[:, ny - 1, :] = 0
+    return flag
+
+
+def poiseuille_profile(y, ny, tau, g):
+    """Force-driven slit flow with the half-way walls at y = 1/2 and y = ny - 3/2."""
+    nu = (tau - 0.5) / 3.0
+    return g / (2.0 * nu) * (y - 0.5) * (ny - 1.5 - y)
+
+
+@pytest.mark.parametrize("tau,tol", [(0.5 + np.sqrt(3.0 / 16.0), 1e-9), (1.0, 3e-3)])
+def test_poiseuille_halfway(oracle_mod, tau, tol):
+    """Slit Poiseuille flow with half-way bounce-back, a physics known answer:
+    at tau = 1/2 + sqrt(3/16) BGK with half-way walls reproduces the
+    parabola exactly (He, Zou, Luo & Dembo, J. Stat. Phys. 87, 1997); other
+    tau leave the known tau-dependent wall slip, here 1.4e-3."""
+    O = oracle_mod
+    ny, g = 20, 1e-6
+    flag = poiseuille_flag(ny)
+    V, start = O.build_V(flag, 1)
+    tab = O.pull_table(V, start, O.BB_HALFWAY)
+    f0, _ = O.init_equilibrium(len(V))
+    mac = O.pull_run(tab, f0, 30000, tau, g)
+    ua = poiseuille_profile(V["y"].astype(float), ny, tau, g)
+    assert np.linalg.norm(mac[1] - ua) / np.linalg.norm(ua) < tol
+    assert np.abs(mac[2:]).max() < 1e-8 * ua.max()  # no cross flow beyond rounding
