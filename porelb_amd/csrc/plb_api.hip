@@ -181,6 +181,7 @@ extern "C" int plb_create(plb_ctx** out, const plb_config* cfg, const plb_node_i
 {
     if (!out) return PLB_ERR_INVALID;
     *out = nullptr;
+    Range range("plb_create");
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (n_nodes < 0 || (n_nodes > 0 && !V) || !V_sub_start) return PLB_ERR_INVALID;
@@ -216,6 +217,7 @@ extern "C" int plb_create_from_flag(plb_ctx** out, const plb_config* cfg, const 
 {
     if (!out) return PLB_ERR_INVALID;
     *out = nullptr;
+    Range range("plb_create_from_flag");
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (!flag || nx <= 0 || ny <= 0 || nz <= 0 || nx > 65535 || ny > 65535 || nz > 65535)
@@ -276,6 +278,7 @@ static unsigned long long read_fault_key(plb_ctx* c, int* rc)
 extern "C" int plb_step_async(plb_ctx* c, int64_t it0, int64_t nsteps)
 {
     if (!c || nsteps < 0 || it0 < 0 || it0 + nsteps >= ((int64_t)1 << 31)) return PLB_ERR_INVALID;
+    Range range("plb_step");
     PLB_HIP(hipSetDevice(c->device));
     if (c->pending_n == 0) {
         c->pending_it0 = it0;
@@ -354,6 +357,7 @@ static const int64_t CHUNK_NODES = 1 << 20;
 extern "C" int plb_read_macros(plb_ctx* c, double* rho, double* ux, double* uy, double* uz)
 {
     if (!c) return PLB_ERR_INVALID;
+    Range range("plb_read_macros");
     PLB_HIP(hipSetDevice(c->device));
     double* outs[4] = {rho, ux, uy, uz};
     const int64_t chunk = std::min<int64_t>(std::max<int64_t>(c->N, 1), 8 * CHUNK_NODES);
@@ -439,6 +443,7 @@ extern "C" int plb_read_table(plb_ctx* c, int32_t* tab)
 extern "C" int plb_reduce(plb_ctx* c, plb_reduction* out)
 {
     if (!c || !out) return PLB_ERR_INVALID;
+    Range range("plb_reduce");
     PLB_HIP(hipSetDevice(c->device));
     double r[2] = {0.0, 0.0};
     if (c->N > 0) {

@@ -232,6 +232,7 @@ extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_ever
                              int64_t* steps_done, int64_t* updates, double* ms)
 {
     if (!b || max_steps < 0 || check_every < 1 || !(tol >= 0.0)) return PLB_ERR_INVALID;
+    Range range("plb_batch_run");
     plb_ctx* c = b->c;
     PLB_HIP(hipSetDevice(c->device));
     int rc = plb_init_equilibrium(c);

@@ -907,6 +907,7 @@ extern "C" int plb_preprocess(const uint8_t* flag, int nx, int ny, int nz, int n
         return PLB_ERR_INVALID;
     if (num_p < 1 || num_p > 65535 || nz / num_p == 0) return PLB_ERR_NUM_P;
     if ((nz - 1) / (nz / num_p) >= num_p) return PLB_ERR_NUM_P;
+    Range range("plb_preprocess");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PLB_ERR_NODEVICE;
     if (device < 0 || device >= ndev) return PLB_ERR_INVALID;

@@ -5,9 +5,11 @@
 //                       (double, or float/half holding f - w_j*rho0)
 //   pull table   tab    Npad*18 int32: >=0 source node, -1 bounce, -2 const
 //                or TILE8: tdesc [T][18] int32 (base >= 0, or -(wide slot+1)),
-//                toff [T][5][64] u32, four u8 offsets per word (0..253
-//                offset, 254 const, 255 bounce),
+//                toff [T][4][64] u32 + toff16 [T][64] u16, u8 offsets
+//                (0..253 offset, 254 const, 255 bounce),
 //                twide [W][64] int32 entries of the wide (tile, direction) pairs
+//                or BITMAP: bm [words] {pore bits, pores before} per 32 voxels
+//                of the node order, vox [Npad] u32 voxel position per node
 //   macros              [4][Npad] double (rho, ux, uy, uz), written on
 //                       request (last step of a call, convergence checks)
 //   perm/inv_perm       internal id <-> V-order id, only for BRICK order
@@ -17,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 
 #include <vector>
@@ -149,6 +152,15 @@ struct plb_ctx {
 void plb_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 
 namespace plb {
+// roctx range around a library call (shows up under rocprofv3 --marker-trace;
+// a no-op without a profiler).
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
+
 // Owner of a temporary device array: freed on every return path.
 template <typename T>
 struct DevBuf {

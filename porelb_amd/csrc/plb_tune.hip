@@ -110,6 +110,7 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
                             plb_tune_result* results, int max_results, int* n_results)
 {
     if (!base || !flag || !best || steps < 1) return PLB_ERR_INVALID;
+    Range range("plb_autotune");
     int rc = check_cfg(base);
     if (rc) return rc;
     if (n_results) *n_results = 0;

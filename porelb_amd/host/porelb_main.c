@@ -16,8 +16,9 @@
  * environment; defaults are the reference's values:
  *   PLB_TAU (0.8)  PLB_GX (1e-3)  PLB_RHO0 (1)  PLB_SAVE (1000)
  *   PLB_PRECISION fp64|fp32|fp16 (fp64)  PLB_BB refcompat|halfway (refcompat)
- *   PLB_LAYOUT soa|aosoa  PLB_ORDER reference|brick|pencil  PLB_BRICK bx,by,bz
- *   PLB_INDEX i32|tile8|bitmap  PLB_BLOCK (256)  PLB_GRAPH (0: steps per graph replay)
+ *   PLB_LAYOUT soa|aosoa (aosoa)  PLB_ORDER reference|brick|pencil (pencil)
+ *   PLB_BRICK bx,by,bz (pencil: nx,64,1)  PLB_INDEX i32|tile8|bitmap (tile8)
+ *   PLB_BLOCK (128 below 4 M pore nodes, else 256)  PLB_GRAPH (0: steps per graph replay)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
  *   PLB_AUTOTUNE cachefile: time index encoding x layout x node order, then
  *   block size x nodes per thread, on this V.bin once; reuse the result from
@@ -121,8 +122,16 @@ int main(int argc, char *argv[])
     cfg.rho0 = env_dbl("PLB_RHO0", cfg.rho0);
     cfg.precision = env_choice("PLB_PRECISION", "fp64", "fp32", "fp16", PLB_FP64);
     cfg.bb_mode = env_choice("PLB_BB", "refcompat", "halfway", NULL, PLB_BB_REFCOMPAT);
-    cfg.layout = env_choice("PLB_LAYOUT", "soa", "aosoa", NULL, PLB_LAYOUT_SOA);
-    cfg.ordering = env_choice("PLB_ORDER", "reference", "brick", "pencil", PLB_ORDER_REFERENCE);
+    /* Default shape: the one measured fastest across the configurations
+     * (AoSoA, pencils of 64 full rows, TILE8; 128 threads on small lattices).
+     * Every shape gives bit-identical results. */
+    cfg.layout = env_choice("PLB_LAYOUT", "soa", "aosoa", NULL, PLB_LAYOUT_AOSOA);
+    cfg.ordering = env_choice("PLB_ORDER", "reference", "brick", "pencil", PLB_ORDER_PENCIL);
+    if (cfg.ordering == PLB_ORDER_PENCIL) {
+        cfg.brick[0] = v.nx;
+        cfg.brick[1] = 64;
+        cfg.brick[2] = 1;
+    }
     {
         const char *b = getenv("PLB_BRICK"); /* "bx,by,bz" */
         if (b && *b && sscanf(b, "%d,%d,%d", &cfg.brick[0], &cfg.brick[1], &cfg.brick[2]) != 3) {
@@ -130,8 +139,8 @@ int main(int argc, char *argv[])
             exit(-10);
         }
     }
-    cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", "bitmap", PLB_INDEX_I32);
-    cfg.block_size = env_int("PLB_BLOCK", cfg.block_size);
+    cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", "bitmap", PLB_INDEX_TILE8);
+    cfg.block_size = env_int("PLB_BLOCK", v.num_node < (1 << 22) ? 128 : 256);
     cfg.graph_steps = env_int("PLB_GRAPH", cfg.graph_steps);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
