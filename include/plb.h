@@ -100,9 +100,8 @@ typedef struct {
     int nodes_per_thread; /* 1, or 2: a wave advances two 64-node tiles at once
                              (block_size <= 512; single lattices only)          */
     int graph_steps;     /* 0: one kernel launch per step; G > 0 (even, <= 4096):
-                            runs of G steps replay one captured HIP graph, which
-                            removes the host launch cost on small lattices
-                            (single lattices only)                             */
+                            runs of G steps replay one captured HIP graph
+                            (single lattices only; bit-identical results)      */
 } plb_config;
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
@@ -212,7 +211,7 @@ int plb_preprocess(const uint8_t *flag, int nx, int ny, int nz, int num_p, int d
 /* ---- batched samples: many independent porous samples of uneven pore
  * count advanced by one launch per step over all active samples, with
  * per-sample convergence stopping.  cfg supplies precision, bounce-back,
- * layout, index encoding, node order (PENCIL uses each sample's own x extent
+ * layout, index encoding (I32 or TILE8), node order (PENCIL uses each sample's own x extent
  * as the pencil width) and block size; tau and gx come per sample.  With FP16 storage and fp16_scale_log2 == 0 the scale is
  * derived from the largest |gx| of the samples. */
 typedef struct {
