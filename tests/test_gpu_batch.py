@@ -13,12 +13,13 @@ def masks(plb):
             plb.mask_spheres((17, 23, 19), 2.0, 4.0, 0.35, 3)]
 
 
-@pytest.mark.parametrize("prec,index,order", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 1, 0), (1, 1, 2), (0, 0, 1)])
-def test_batch_equals_single_runs(plb, prec, index, order):
+@pytest.mark.parametrize("prec,index,order,bb", [(0, 0, 0, 1), (1, 0, 0, 1), (1, 1, 0, 1), (2, 1, 0, 1),
+                                                 (1, 1, 2, 1), (0, 0, 1, 1), (0, 1, 2, 0), (1, 1, 2, 0)])
+def test_batch_equals_single_runs(plb, prec, index, order, bb):
     fl = masks(plb)
     taus, gxs = [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6]
     brick = (8, 4, 2) if order == 1 else (64, 4, 1)
-    b = plb.Batch(fl, taus, gxs, precision=prec, bb_mode=plb.BB_HALFWAY, index_kind=index, layout=index,
+    b = plb.Batch(fl, taus, gxs, precision=prec, bb_mode=bb, index_kind=index, layout=index,
                   fp16_scale_log2=20, ordering=order, brick=brick)
     steps, upd, ms = b.run(300, 100, 0.0)   # tol 0: nobody stops early
     assert list(steps) == [300, 300, 300]
@@ -27,7 +28,7 @@ def test_batch_equals_single_runs(plb, prec, index, order):
     for s, f in enumerate(fl):
         sbrick = (f.shape[2],) + brick[1:] if order == 2 else brick
         with plb.Lattice.from_flag(f, tau=taus[s], gx=gxs[s], precision=prec,
-                                   bb_mode=plb.BB_HALFWAY, index_kind=index, layout=index,
+                                   bb_mode=bb, index_kind=index, layout=index,
                                    fp16_scale_log2=20, ordering=order, brick=sbrick) as lat:
             assert lat.N == b.nodes[s]
             lat.step(0, 300)
