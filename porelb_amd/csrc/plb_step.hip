@@ -3,9 +3,10 @@
 // reductions.
 //
 // One kernel per time step, one thread per pore node:
-//   1. 18 pull-table entries (int32, or TILE8 bases + packed u8 offsets);
+//   1. 18 pull-table entries (int32, TILE8 bases + packed u8 offsets, or
+//      derived from the BITMAP index);
 //   2. 19 population gathers issued back to back (stream: f[j][src];
-//      bounce: f[re[j]][self]; const: no data needed);
+//      bounce and const: f[re[j]][self], const replaced after the load);
 //   3. collision with body force (porelb.c:318-365);
 //   4. 19 coalesced non-temporal stores into the other ping-pong buffer.
 // This fuses stream_inner_nodes, update_recv_dist and collision
