@@ -23,11 +23,14 @@
  *   PLB_AUTOTUNE cachefile: time index encoding x layout x node order, then
  *   block size x nodes per thread, on this V.bin once; reuse the result from
  *   cachefile after
+ *   PLB_ASYNC_SAVE 1|0 (1): part files are written by a worker thread from one
+ *   of two host buffers while the GPU keeps stepping (same files)
  *   PLB_CHECKPOINT file: populations written after the last step
  *   PLB_RESTART file: start from such a checkpoint (fills the empty
  *   save_state/recovery_state stubs of porelb.c:367-373)
  */
 #define _POSIX_C_SOURCE 199309L
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -66,21 +69,64 @@ static double wtime(void)
 
 #define CKPT_MAGIC 0x504c4243u /* "PLBC" */
 
-static void save_all(plb_ctx *c, const plb_vfile *v, int it, double *buf)
+/* save_data (porelb.c:375-391) for every rank: the macros of step `it` go
+ * to one host buffer, and its part files are written either at once or by a
+ * worker thread while the GPU runs the next steps.  Two buffers alternate; a
+ * buffer is reused only after its writer has finished. */
+typedef struct {
+    const plb_vfile *v;
+    double *buf;
+    int it;
+    int failed;
+    int running;
+    pthread_t th;
+} plb_saver;
+
+static void *save_worker(void *arg)
 {
+    plb_saver *s = (plb_saver *)arg;
+    const plb_vfile *v = s->v;
     const int64_t N = v->num_node;
-    if (plb_read_macros(c, buf, buf + N, buf + 2 * N, buf + 3 * N)) {
+    const double *buf = s->buf;
+    for (int p = 0; p < v->num_p && !s->failed; p++) {
+        int64_t b = v->V_sub_start[p];
+        int64_t e = p + 1 < v->num_p ? v->V_sub_start[p + 1] : N;
+        if (plb_write_part(p, s->it, e - b, buf + b, buf + N + b, buf + 2 * N + b, buf + 3 * N + b))
+            s->failed = 1;
+    }
+    return NULL;
+}
+
+static void saver_wait(plb_saver *s)
+{
+    if (s->running) {
+        pthread_join(s->th, NULL);
+        s->running = 0;
+    }
+    if (s->failed) {
+        fprintf(stderr, "Macro binary file open error!\n");
+        exit(-3);
+    }
+}
+
+static void save_all(plb_ctx *c, plb_saver sv[2], int *k, int async, int it)
+{
+    plb_saver *s = &sv[*k];
+    saver_wait(s);
+    const int64_t N = s->v->num_node;
+    if (plb_read_macros(c, s->buf, s->buf + N, s->buf + 2 * N, s->buf + 3 * N)) {
         fprintf(stderr, "Macro read error!\n");
         exit(-3);
     }
-    for (int p = 0; p < v->num_p; p++) {
-        int64_t b = v->V_sub_start[p];
-        int64_t e = p + 1 < v->num_p ? v->V_sub_start[p + 1] : N;
-        if (plb_write_part(p, it, e - b, buf + b, buf + N + b, buf + 2 * N + b, buf + 3 * N + b)) {
-            fprintf(stderr, "Macro binary file open error!\n");
-            exit(-3);
-        }
+    s->it = it;
+    s->failed = 0;
+    if (async && pthread_create(&s->th, NULL, save_worker, s) == 0) {
+        s->running = 1;
+    } else {
+        save_worker(s);
+        saver_wait(s);
     }
+    if (async) *k ^= 1;
 }
 
 int main(int argc, char *argv[])
@@ -213,7 +259,18 @@ int main(int argc, char *argv[])
         free(f);
     }
 
-    double *buf = (double *)malloc(sizeof(double) * 4 * (size_t)(N ? N : 1));
+    const int async = env_int("PLB_ASYNC_SAVE", 1);
+    plb_saver sv[2];
+    memset(sv, 0, sizeof sv);
+    for (int b = 0; b < (async ? 2 : 1); b++) {
+        sv[b].v = &v;
+        sv[b].buf = (double *)malloc(sizeof(double) * 4 * (size_t)(N ? N : 1));
+        if (!sv[b].buf) {
+            fprintf(stderr, "Macro memory allocation error!\n");
+            exit(-4);
+        }
+    }
+    int kbuf = 0;
     double t1 = wtime();
     int it = it0;
     while (it < it_max) {
@@ -228,15 +285,19 @@ int main(int argc, char *argv[])
                 for (int s = it; s < ft.step; s++) printf(">>> Step : %20d \n", s);
             fprintf(stderr, "%d >>> at %d (%d, %d, %d),  Macro denstity  = %g , < 0.5 !\n", ft.pid,
                     (int)ft.local_id, ft.x, ft.y, ft.z, ft.rho);
+            saver_wait(&sv[0]);
+            saver_wait(&sv[1]);
             exit(-6);
         }
         if (rc) {
             fprintf(stderr, ">>> libplb: %s\n", plb_strerror(rc));
+            saver_wait(&sv[0]);
+            saver_wait(&sv[1]);
             exit(rc);
         }
         if (print)
             for (int s = it; s <= last; s++) printf(">>> Step : %20d \n", s);
-        if (last % save_every == 0) save_all(c, &v, last, buf);
+        if (last % save_every == 0) save_all(c, sv, &kbuf, async, last);
         it = last + 1;
     }
     double t2 = wtime();
@@ -244,7 +305,9 @@ int main(int argc, char *argv[])
     if (t2 > t1 && it_max > it0)
         printf("MLUPS %.3f (%lld pore nodes x %d steps)\n",
                (double)N * (it_max - it0) / (t2 - t1) / 1e6, (long long)N, it_max - it0);
-    save_all(c, &v, it_max, buf);
+    save_all(c, sv, &kbuf, async, it_max);
+    saver_wait(&sv[0]);
+    saver_wait(&sv[1]);
 
     const char *ckpt = getenv("PLB_CHECKPOINT");
     if (ckpt && *ckpt) {
@@ -262,7 +325,8 @@ int main(int argc, char *argv[])
         fclose(fp);
         free(f);
     }
-    free(buf);
+    free(sv[0].buf);
+    free(sv[1].buf);
     plb_destroy(c);
     plb_free_vfile(&v);
     return 0;

@@ -174,3 +174,23 @@ def test_porelb_fp32_pencil_tile8_close_to_fp64(tmp_path):
     ux64, ux32 = out["fp64"][1], out["fp32"][1]
     assert np.abs(out["fp32"][0] - out["fp64"][0]).max() < 1e-6
     assert np.linalg.norm(ux32 - ux64) / np.linalg.norm(ux64) < 1e-4
+
+
+def test_porelb_async_saves_identical(tmp_path):
+    """Part files written by the save worker thread (PLB_ASYNC_SAVE=1, the
+    default) equal those written inline, at every save step."""
+    flag = (np.random.default_rng(31).random((16, 12, 14)) < 0.6).astype(np.uint8)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-5", "PLB_PRINT": "0", "PLB_SAVE": "7"}
+    for mode in ("0", "1"):
+        wd = tmp_path / f"async{mode}"
+        wd.mkdir()
+        os.link(tmp_path / "V.bin", wd / "V.bin")
+        r = run([os.path.join(BIN, "porelb.x"), "50"], wd, dict(env, PLB_ASYNC_SAVE=mode))
+        assert r.returncode == 0, r.stderr
+    for it in list(range(0, 50, 7)) + [50]:
+        a = read_parts(tmp_path / "async0", 2, it)
+        b = read_parts(tmp_path / "async1", 2, it)
+        assert np.array_equal(a, b), it
