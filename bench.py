@@ -289,7 +289,8 @@ def study_outputs(P, reps=5):
     """The output side of a step at 1024^3 (FP32 bench shape): the FP64
     reduction (plb_reduce: populations and u_x read once, 19*4 + 8 B per node)
     and the macro read-back that save_data needs (plb_read_macros: 4 x 8 B per
-    node gathered to V order and copied to the host)."""
+    node gathered to V order and copied to pageable, then to page-locked host
+    memory)."""
     w = WORKLOADS["spheres1024-fp32"]
     flag = make_mask(P, w)
     with P.Lattice.from_flag(flag, 1, tau=w["tau"], gx=w["gx"], precision=1, bb_mode=1, index_kind=1, layout=1,
@@ -308,11 +309,21 @@ def study_outputs(P, reps=5):
             lat.read_macros()
             t.append(time.perf_counter() - t0)
         mac = min(t)
+        pinned = P.host_array((4, n))
+        t = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            lat.read_macros(out=pinned)
+            t.append(time.perf_counter() - t0)
+        mac_pinned = min(t)
+        del pinned
     out = {"study": "outputs", "workload": w["label"], "pore_nodes": n,
            "reduce_s": round(red, 5), "reduce_bytes": n * (19 * 4 + 8),
            "reduce_gbs": round(n * (19 * 4 + 8) / red / 1e9, 1),
            "reduce_roofline_frac": round(n * (19 * 4 + 8) / red / (HBM_PEAK_GBS * 1e9), 3),
-           "read_macros_s": round(mac, 4), "read_macros_host_gbs": round(n * 32 / mac / 1e9, 2)}
+           "read_macros_s": round(mac, 4), "read_macros_host_gbs": round(n * 32 / mac / 1e9, 2),
+           "read_macros_pinned_s": round(mac_pinned, 4),
+           "read_macros_pinned_gbs": round(n * 32 / mac_pinned / 1e9, 2)}
     print(json.dumps(out), flush=True)
 
 

@@ -201,6 +201,13 @@ double plb_bytes_per_update(const plb_ctx *c);
 
 const char *plb_strerror(int code);
 
+/* Page-locked host memory (new; the reference writes from malloc'd arrays,
+ * porelb.c:375-391).  plb_read_macros / plb_read_populations copy into such a
+ * buffer by DMA at the full PCIe rate instead of through a staging copy.
+ * Returns NULL on failure; release with plb_host_free. */
+void *plb_host_alloc(size_t bytes);
+void plb_host_free(void *p);
+
 /* ---- preprocess on the device (preprocess.c:29-164 build_V): flag ->
  * V records (V order) and V_sub_start.  Call with V_out == NULL to get the
  * pore count only.  Fields preprocess.c leaves uninitialised (SOLID pid/id,

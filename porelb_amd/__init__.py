@@ -104,6 +104,8 @@ def _lib():
             "plb_last_fault": ([p, p], i32),
             "plb_bytes_per_update": ([p], d),
             "plb_strerror": ([i32], C.c_char_p),
+            "plb_host_alloc": ([C.c_size_t], C.c_void_p),
+            "plb_host_free": ([C.c_void_p], None),
             "plb_preprocess": ([p, i32, i32, i32, i32, i32, p, p, p], i32),
             "plb_batch_create": ([p, p, p, i32], i32),
             "plb_batch_run": ([p, i64, i64, d, p, p, p], i32),
@@ -194,6 +196,20 @@ def preprocess(flag, num_p, device=0):
     return V, start
 
 
+def host_array(shape, dtype=np.float64):
+    """A numpy array in page-locked host memory (plb_host_alloc), freed when
+    the array is garbage-collected; read-backs into it run at the PCIe rate."""
+    import weakref
+
+    nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    p = _lib().plb_host_alloc(nbytes)
+    if not p:
+        raise MemoryError(f"plb_host_alloc({nbytes}) failed")
+    raw = (C.c_char * max(nbytes, 1)).from_address(p)
+    weakref.finalize(raw, _lib().plb_host_free, p)
+    return np.frombuffer(raw, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+
 # ------------------------------------------------------------------ lattice
 class Lattice:
     """One pore-node lattice resident on the GPU (a plb_ctx)."""
@@ -257,8 +273,11 @@ class Lattice:
         _check(_lib().plb_step_timed(self._h, it0, nsteps, C.byref(ms)), "plb_step_timed")
         return ms.value
 
-    def read_macros(self):
-        out = np.zeros((4, self.N))
+    def read_macros(self, out=None):
+        """(4, N) rho, ux, uy, uz in V order; `out` may be a host_array."""
+        if out is None:
+            out = np.zeros((4, self.N))
+        assert out.shape == (4, self.N) and out.dtype == np.float64 and out.flags.c_contiguous
         _check(_lib().plb_read_macros(self._h, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _ptr(out[3])),
                "plb_read_macros")
         return out

@@ -346,6 +346,18 @@ extern "C" int plb_step_timed(plb_ctx* c, int64_t it0, int64_t nsteps, double* m
 
 extern "C" int64_t plb_num_nodes(const plb_ctx* c) { return c ? c->N : -1; }
 
+extern "C" void* plb_host_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+extern "C" void plb_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
 extern "C" double plb_bytes_per_update(const plb_ctx* c)
 {
     if (!c) return 0.0;

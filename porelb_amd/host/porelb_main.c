@@ -264,7 +264,8 @@ int main(int argc, char *argv[])
     memset(sv, 0, sizeof sv);
     for (int b = 0; b < (async ? 2 : 1); b++) {
         sv[b].v = &v;
-        sv[b].buf = (double *)malloc(sizeof(double) * 4 * (size_t)(N ? N : 1));
+        /* page-locked, so the read-back runs at the PCIe rate */
+        sv[b].buf = (double *)plb_host_alloc(sizeof(double) * 4 * (size_t)(N ? N : 1));
         if (!sv[b].buf) {
             fprintf(stderr, "Macro memory allocation error!\n");
             exit(-4);
@@ -325,8 +326,8 @@ int main(int argc, char *argv[])
         fclose(fp);
         free(f);
     }
-    free(sv[0].buf);
-    free(sv[1].buf);
+    plb_host_free(sv[0].buf);
+    plb_host_free(sv[1].buf);
     plb_destroy(c);
     plb_free_vfile(&v);
     return 0;

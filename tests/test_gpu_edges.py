@@ -104,3 +104,13 @@ def test_bad_V_records_rejected(plb):
     with pytest.raises(plb.PlbError) as ei:
         plb.Lattice.from_V(bad, start, flag.shape)
     assert ei.value.code == -4
+
+
+def test_read_into_page_locked_buffer(plb):
+    """plb_host_alloc buffers receive the same macros as pageable arrays."""
+    flag = (np.random.default_rng(8).random((10, 9, 11)) < 0.6).astype(np.uint8)
+    with plb.Lattice.from_flag(flag, tau=0.8, gx=1e-4) as lat:
+        lat.step(0, 20)
+        a = lat.read_macros()
+        b = lat.read_macros(out=plb.host_array((4, lat.N)))
+        assert np.array_equal(a, b)
