@@ -143,3 +143,19 @@ def test_postprocess_errors(tmp_path, plb):
     assert r.returncode == (-3) & 0xFF and "out of range" in r.stderr
     r = run([os.path.join(BIN, "postprocess.x"), "K", "10"], tmp_path)  # part file missing
     assert r.returncode == (-1) & 0xFF and "Part file openning error!" in r.stderr
+
+
+def test_porelb_bad_environment_values(tmp_path, plb):
+    """Unknown values of the configuration variables stop porelb.x with -10
+    before any GPU work."""
+    import oracle as O
+
+    flag = np.ones((4, 4, 4), dtype=np.uint8)
+    V, start = O.build_V(flag, 1)
+    plb.write_vbin(str(tmp_path / "V.bin"), V, start, flag.shape)
+    for env, text in [({"PLB_PRECISION": "fp8"}, "bad value PLB_PRECISION=fp8"),
+                      ({"PLB_INDEX": "tile4"}, "bad value PLB_INDEX=tile4"),
+                      ({"PLB_BRICK": "4,4"}, "bad value PLB_BRICK=4,4"),
+                      ({"PLB_SAVE": "0"}, "PLB_SAVE must be >= 1")]:
+        r = run([os.path.join(BIN, "porelb.x"), "10"], tmp_path, env=env)
+        assert r.returncode != 0 and text in r.stderr, (env, r.stderr)
