@@ -246,8 +246,8 @@ int main(int argc, char *argv[])
         FILE *fp = fopen(restart, "rb");
         uint32_t magic = 0;
         int64_t hdr[2] = {0, 0};
-        double *f = (double *)malloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
-        if (!fp || fread(&magic, 4, 1, fp) != 1 || magic != CKPT_MAGIC ||
+        double *f = (double *)plb_host_alloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
+        if (!f || !fp || fread(&magic, 4, 1, fp) != 1 || magic != CKPT_MAGIC ||
             fread(hdr, 8, 2, fp) != 2 || hdr[1] != N ||
             fread(f, sizeof(double) * 19, (size_t)N, fp) != (size_t)N) {
             fprintf(stderr, ">>> cannot read checkpoint %s\n", restart);
@@ -256,7 +256,7 @@ int main(int argc, char *argv[])
         fclose(fp);
         it0 = (int)hdr[0];
         if (plb_write_populations(c, f)) exit(-5);
-        free(f);
+        plb_host_free(f);
     }
 
     const int async = env_int("PLB_ASYNC_SAVE", 1);
@@ -312,11 +312,11 @@ int main(int argc, char *argv[])
 
     const char *ckpt = getenv("PLB_CHECKPOINT");
     if (ckpt && *ckpt) {
-        double *f = (double *)malloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
+        double *f = (double *)plb_host_alloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
         FILE *fp = fopen(ckpt, "wb");
         uint32_t magic = CKPT_MAGIC;
         int64_t hdr[2] = {it_max > it0 ? it_max : it0, N};
-        if (!fp || plb_read_populations(c, f)) {
+        if (!f || !fp || plb_read_populations(c, f)) {
             fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
             exit(-3);
         }
@@ -324,7 +324,7 @@ int main(int argc, char *argv[])
         fwrite(hdr, 8, 2, fp);
         fwrite(f, sizeof(double) * 19, (size_t)N, fp);
         fclose(fp);
-        free(f);
+        plb_host_free(f);
     }
     plb_host_free(sv[0].buf);
     plb_host_free(sv[1].buf);
