@@ -122,7 +122,7 @@ def cpu_baseline(P, w, max_seconds=20.0):
     npore = int(flag.sum())
     if O.have_reference_build() and os.path.exists(O.MPIRUN):
         per_rank_mlups = 8.0
-        steps = int(max(10, min(400, max_seconds * ranks * per_rank_mlups * 1e6 / npore)))
+        steps = int(max(10, min(1000, max_seconds * ranks * per_rank_mlups * 1e6 / npore)))
         wd = tempfile.mkdtemp(prefix="plbcpu_")
         log(f"cpu baseline: reference porelb.x, mpirun -np {ranks}, {npore} nodes, {steps} steps")
         r = O.run_reference(flag, ranks, steps, w["tau"], w["gx"], save_every=1000, print_steps=1,
