@@ -67,7 +67,7 @@ int plb_write_vfile(const char *path, const plb_vfile *v)
     int ok = fwrite(hdr, sizeof(int32_t), 5, fp) == 5 &&
              fwrite(v->V_sub_start, sizeof(int32_t), (size_t)v->num_p, fp) == (size_t)v->num_p &&
              fwrite(v->V, sizeof(plb_node_info), (size_t)v->num_node, fp) == (size_t)v->num_node;
-    fclose(fp);
+    ok &= fclose(fp) == 0;
     return ok ? 0 : -1;
 }
 
@@ -86,12 +86,13 @@ int plb_write_part(int pid, int it, int64_t n, const double *rho, const double *
     snprintf(name, sizeof name, "part_%03d_%07d.bin", pid, it);
     FILE *fp = fopen(name, "wb");
     if (!fp) return -3;
-    fwrite(rho, sizeof(double), (size_t)n, fp);
-    fwrite(ux, sizeof(double), (size_t)n, fp);
-    fwrite(uy, sizeof(double), (size_t)n, fp);
-    fwrite(uz, sizeof(double), (size_t)n, fp);
-    fclose(fp);
-    return 0;
+    /* the reference leaves short writes unchecked; a full disk is an error here */
+    int ok = fwrite(rho, sizeof(double), (size_t)n, fp) == (size_t)n &&
+             fwrite(ux, sizeof(double), (size_t)n, fp) == (size_t)n &&
+             fwrite(uy, sizeof(double), (size_t)n, fp) == (size_t)n &&
+             fwrite(uz, sizeof(double), (size_t)n, fp) == (size_t)n;
+    ok &= fclose(fp) == 0;
+    return ok ? 0 : -3;
 }
 
 int plb_read_part(int pid, int it, int64_t n, double *rho, double *ux, double *uy, double *uz)

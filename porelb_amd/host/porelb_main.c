@@ -320,11 +320,14 @@ int main(int argc, char *argv[])
             fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
             exit(-3);
         }
-        fwrite(&magic, 4, 1, fp);
-        fwrite(hdr, 8, 2, fp);
-        fwrite(f, sizeof(double) * 19, (size_t)N, fp);
-        fclose(fp);
+        int bad = fwrite(&magic, 4, 1, fp) != 1 || fwrite(hdr, 8, 2, fp) != 2 ||
+                  fwrite(f, sizeof(double) * 19, (size_t)N, fp) != (size_t)N;
+        bad |= fclose(fp) != 0;
         plb_host_free(f);
+        if (bad) {
+            fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
+            exit(-3);
+        }
     }
     plb_host_free(sv[0].buf);
     plb_host_free(sv[1].buf);

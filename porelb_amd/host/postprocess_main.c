@@ -109,7 +109,10 @@ static void slice(char axis, int n, int t)
                         (double)i, rho[loc], u[loc], v_[loc], w[loc], (double)flag[loc]);
             }
     }
-    fclose(fp);
+    if (fclose(fp) != 0) {
+        fprintf(stderr, "Slice file write error!\n");
+        exit(-3);
+    }
 }
 
 static void full(int t)
@@ -133,7 +136,10 @@ static void full(int t)
                        : var == 4 ? u[loc] : var == 5 ? v_[loc] : var == 6 ? w[loc] : flag[loc];
             fprintf(fp, "%.14e\n", val);
         }
-    fclose(fp);
+    if (fclose(fp) != 0) {
+        fprintf(stderr, "Full file write error!\n");
+        exit(-3);
+    }
 }
 
 int main(int argc, char *argv[])

@@ -145,6 +145,29 @@ def test_postprocess_errors(tmp_path, plb):
     assert r.returncode == (-1) & 0xFF and "Part file openning error!" in r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists("/dev/full"), reason="needs /dev/full")
+def test_postprocess_full_disk_is_an_error(tmp_path, plb):
+    """An output that cannot be written (here /dev/full, which fails every
+    write with ENOSPC) ends postprocess.x with -3 instead of a truncated file."""
+    import oracle as O
+
+    flag = np.ones((3, 4, 5), dtype=np.uint8)
+    V, start = O.build_V(flag, 1)
+    plb.write_vbin(str(tmp_path / "V.bin"), V, start, flag.shape)
+    flag.tofile(tmp_path / "flag.bin")
+    np.ones((4, len(V))).tofile(tmp_path / "part_000_0000010.bin")
+    r = run([os.path.join(BIN, "postprocess.x"), "X", "1", "10"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    for args, name, text in [(["X", "1", "10"], "Slice_X_0000010.dat", "Slice file write error!"),
+                             (["A", "10"], "full_0000010.dat", "Full file write error!")]:
+        out = tmp_path / name
+        if out.exists():
+            out.unlink()
+        out.symlink_to("/dev/full")
+        r = run([os.path.join(BIN, "postprocess.x")] + args, tmp_path)
+        assert r.returncode == (-3) & 0xFF and text in r.stderr, (args, r.returncode, r.stderr)
+
+
 def test_porelb_bad_environment_values(tmp_path, plb):
     """Unknown values of the configuration variables stop porelb.x with -10
     before any GPU work."""

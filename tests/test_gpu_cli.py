@@ -194,3 +194,23 @@ def test_porelb_async_saves_identical(tmp_path):
         a = read_parts(tmp_path / "async0", 2, it)
         b = read_parts(tmp_path / "async1", 2, it)
         assert np.array_equal(a, b), it
+
+
+@pytest.mark.skipif(not os.path.exists("/dev/full"), reason="needs /dev/full")
+@pytest.mark.parametrize("mode", ["0", "1"])
+def test_porelb_part_file_write_failure(tmp_path, mode):
+    """A part file that cannot be written (here /dev/full, which fails every
+    write) ends porelb.x with the reference's save_data code -3
+    (porelb.c:375-391), from the inline writer and from the worker thread;
+    so does a checkpoint that cannot be written."""
+    flag = (np.random.default_rng(32).random((8, 6, 7)) < 0.6).astype(np.uint8)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "1", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-5", "PLB_PRINT": "0", "PLB_SAVE": "5", "PLB_ASYNC_SAVE": mode}
+    (tmp_path / "part_000_0000010.bin").symlink_to("/dev/full")
+    r = run([os.path.join(BIN, "porelb.x"), "20"], tmp_path, env)
+    assert r.returncode == (-3) & 0xFF and "Macro binary file open error!" in r.stderr, (r.returncode, r.stderr)
+    (tmp_path / "part_000_0000010.bin").unlink()
+    r = run([os.path.join(BIN, "porelb.x"), "20"], tmp_path, dict(env, PLB_CHECKPOINT="/dev/full"))
+    assert r.returncode == (-3) & 0xFF and "cannot write checkpoint" in r.stderr, (r.returncode, r.stderr)
