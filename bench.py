@@ -41,7 +41,7 @@ SPHERES1024 = dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0,
                    tau=0.6, gx=5e-7, label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7")
 WORKLOADS = {
     "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=2,
-                             brick=(1024, 128, 1), block=256),
+                             brick=(1024, 128, 1), block=256, waves=5),
     "spheres1024-fp16": dict(SPHERES1024, precision=2, bb=1, index=1, layout=1, ordering=2,
                              brick=(1024, 64, 1), block=128),
     "spheres1024-fp64": dict(SPHERES1024, precision=0, bb=0, index=0, layout=1, ordering=2,
@@ -57,7 +57,7 @@ WORKLOADS = {
 }
 # The plain FP64 path the reduced-precision paths are compared with: the
 # reference bounce order and the int32 pull table, in its fastest launch shape.
-FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=2, brick=(1024, 64, 1), block=256)
+FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=2, brick=(1024, 64, 1), block=256, waves=5)
 PREC_NAME = {0: "f64", 1: "f32-store/f32-collide", 2: "f16-store/f32-collide"}
 
 
@@ -87,9 +87,10 @@ def make_mask(P, w):
 
 
 def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256, index=0,
-              brick=(16, 4, 4)):
+              brick=(16, 4, 4), waves=0):
     cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=precision, bb_mode=bb, layout=layout,
-                           ordering=ordering, block_size=block, index_kind=index, brick=brick)
+                           ordering=ordering, block_size=block, index_kind=index, brick=brick,
+                           waves_per_simd=waves)
     t0 = time.perf_counter()
     lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
     log(f"lattice built: {lat.N} pore nodes, precision {precision}, {time.perf_counter() - t0:.1f} s")
@@ -381,6 +382,7 @@ def main():
     ap.add_argument("--ordering", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
+    ap.add_argument("--waves", type=int, default=None, help="step-kernel waves per SIMD (0 = no cap)")
     ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess", "outputs"], default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
@@ -427,8 +429,10 @@ def main():
 
     shape = {k: (getattr(args, k) if getattr(args, k) is not None else w[k])
              for k in ("layout", "ordering", "block", "index")}
+    shape["waves"] = args.waves if args.waves is not None else w.get("waves", 0)
     main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, shape["layout"],
-                         shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)))
+                         shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)),
+                         shape["waves"])
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
     avg_ms = main_run["event_ms"] / steps
@@ -451,14 +455,15 @@ def main():
         fp = FP64_PLAIN
         brick = (w["dims"][2],) + fp["brick"][1:]
         fp64 = time_path(P, flag, w, 0, 0, max(20, steps // 2), max(5, args.warmup // 2), fp["layout"],
-                         fp["ordering"], fp["block"], fp["index"], brick)
+                         fp["ordering"], fp["block"], fp["index"], brick, fp["waves"])
         fp64_mlups = fp64["nodes"] * max(20, steps // 2) / fp64["wall_s"] / 1e6
         secondary = {"fp64_plain_mlups": round(fp64_mlups, 1),
                      "fp64_plain_roofline_frac": round((2 * 19 * 8 + 19 * 4) * fp64["nodes"] /
                                                        (fp64["event_ms"] / max(20, steps // 2) * 1e-3) /
                                                        1e9 / HBM_PEAK_GBS, 4),
                      "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3),
-                     "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads"}
+                     "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads, "
+                                         "at most 5 waves per SIMD"}
 
     out = {
         "metric": "pore-node MLUPS",
@@ -478,7 +483,8 @@ def main():
                    "pull_table": "TILE8" if shape["index"] else "int32",
                    "layout": "AoSoA64" if shape["layout"] else "SoA",
                    "node_order": ["reference", "bricks", "pencils"][shape["ordering"]],
-                   "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"]},
+                   "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"],
+                   "waves_per_simd": shape["waves"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_update": algo_bpu, "kernel_ms": round(avg_ms, 4),

@@ -30,9 +30,11 @@ typedef struct {
  * kept) on the mask, `steps` timed steps each, and writes the fastest
  * configuration to *best.  Stage 1: index encoding (I32, TILE8, and BITMAP
  * for half-way bounce-back) x layout (SOA, AOSOA) x node order (reference;
- * 8^3 bricks; pencils of 64 and 128 full rows) at 256 threads.  Stage 2:
- * block size (128, 256, 512) x nodes per thread (1, 2) for the stage-1
- * winner.  All index encodings give bit-identical results.  When cache_path
+ * 8^3 bricks; pencils of 64 and 128 full rows) at 256 threads, each on a
+ * context of its own.  Stage 2: block size (128, 256, 512) x nodes per thread
+ * (1, 2) x waves per SIMD (no cap, 6, 5, 4) for the stage-1 winner, all timed
+ * on one context, since they change only the launch.  All candidates give
+ * bit-identical results.  When cache_path
  * is not NULL, a result stored there for the same mask (64-bit hash of the
  * voxels and dimensions), precision and bounce-back mode is returned without
  * timing, and new results are appended (one text line per result).

@@ -52,7 +52,7 @@ class Config(C.Structure):
                 ("precision", C.c_int), ("bb_mode", C.c_int), ("layout", C.c_int),
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
                 ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int),
-                ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int)]
+                ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int), ("waves_per_simd", C.c_int)]
 
 
 class Reduction(C.Structure):
@@ -325,8 +325,8 @@ def copy_bandwidth(nbytes=8 << 30, iters=5, device=0):
 
 def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
     """Fastest (index encoding, layout, node order, block size, nodes per
-    thread) for this mask, by plb_autotune's two-stage search; returns
-    (Config, results)."""
+    thread, waves per SIMD) for this mask, by plb_autotune's two-stage search;
+    returns (Config, results)."""
     cfg = cfg or default_config(**kw)
     flag = np.ascontiguousarray(flag, dtype=np.uint8)
     nz, ny, nx = flag.shape
@@ -340,7 +340,8 @@ def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
     for r in res[:n.value]:
         c = r.cfg
         out.append(dict(index_kind=c.index_kind, layout=c.layout, ordering=c.ordering, brick=tuple(c.brick),
-                        block=c.block_size, npt=c.nodes_per_thread, mlups=r.mlups, ms=r.ms))
+                        block=c.block_size, npt=c.nodes_per_thread, waves=c.waves_per_simd,
+                        mlups=r.mlups, ms=r.ms))
     return best, out
 
 

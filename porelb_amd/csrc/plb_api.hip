@@ -50,6 +50,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->index_kind = PLB_INDEX_I32;
     cfg->nodes_per_thread = 1;
     cfg->graph_steps = 0;
+    cfg->waves_per_simd = 0;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -102,6 +103,7 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->nodes_per_thread != 1 && cfg->nodes_per_thread != 2) return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread == 2 && cfg->block_size > 512) return PLB_ERR_INVALID;
     if (cfg->graph_steps < 0 || cfg->graph_steps > 4096 || cfg->graph_steps % 2) return PLB_ERR_INVALID;
+    if (cfg->waves_per_simd < 0 || cfg->waves_per_simd > 8) return PLB_ERR_INVALID;
     return PLB_OK;
 }
 
@@ -131,6 +133,8 @@ int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, in
     if (cfg->device < 0 || cfg->device >= ndev) return PLB_ERR_INVALID;
     c->device = cfg->device;
     PLB_HIP(hipSetDevice(c->device));
+    PLB_HIP(hipDeviceGetAttribute(&c->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                  c->device));
     PLB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PLB_HIP(hipEventCreate(&c->ev0));
     PLB_HIP(hipEventCreate(&c->ev1));

@@ -104,6 +104,7 @@ __host__ __device__ __forceinline__ uint64_t pack_xyz(uint64_t x, uint64_t y, ui
 struct plb_ctx {
     plb_config cfg;
     int device = 0;
+    int lds_per_cu = 0;         // bytes of LDS per CU (occupancy cap of the step kernel)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int64_t N = 0, Npad = 0;

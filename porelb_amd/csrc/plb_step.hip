@@ -718,6 +718,19 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     return a;
 }
 
+// Dynamic LDS per block that caps the step kernel at cfg.waves_per_simd waves
+// per SIMD (the kernel itself uses no LDS): with w waves on each of a CU's 4
+// SIMDs the CU holds 4*w*64/bs blocks, so each block reserves that share of
+// the CU's LDS, rounded down to 512 bytes (at most 64 KB).  0 = no cap.
+static unsigned step_lds_bytes(const plb_ctx* c, int bs)
+{
+    const int w = c->cfg.waves_per_simd;
+    if (w <= 0 || w >= 8 || c->lds_per_cu <= 0) return 0u;
+    const int blocks = std::max(1, 4 * w * 64 / bs);
+    const long share = std::min<long>(65536, (long)c->lds_per_cu / blocks);
+    return (unsigned)(share / 512 * 512);
+}
+
 int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_base)
 {
     if (c->N == 0) {  // a mask without pores: nothing to update
@@ -727,13 +740,14 @@ int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_bas
     StepArgs a = make_args(c, it, write_macros);
     a.it_base = it_base;
     const int bs = c->cfg.block_size;
+    const unsigned lds = step_lds_bytes(c, bs);
     if (c->cfg.nodes_per_thread == 2) {
         const int64_t threads = (c->N + 127) / 128 * 64;
         PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step2,
-                      dim3(nblk(threads, bs)), dim3(bs), 0, c->stream, a);
+                      dim3(nblk(threads, bs)), dim3(bs), lds, c->stream, a);
     } else {
         PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step, dim3(nblk(c->N, bs)),
-                      dim3(bs), 0, c->stream, a);
+                      dim3(bs), lds, c->stream, a);
     }
     PLB_HIP(hipGetLastError());
     c->cur ^= 1;
@@ -790,7 +804,8 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
         b.omega = omega;
         b.gx = gx;
         PLB_DISPATCH2(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step_batch,
-                      dim3(n_active), dim3(c->cfg.block_size), 0, c->stream, a, b);
+                      dim3(n_active), dim3(c->cfg.block_size), step_lds_bytes(c, c->cfg.block_size),
+                      c->stream, a, b);
         PLB_HIP(hipGetLastError());
     }
     c->cur ^= 1;

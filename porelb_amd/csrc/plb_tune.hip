@@ -86,9 +86,9 @@ static bool cache_lookup(const char* path, uint64_t key, int prec, int bb, plb_c
     bool found = false;
     while (fgets(line, sizeof line, fp)) {
         unsigned long long k;
-        int p, b, layout, order, bx, by, bz, bs, npt, idx;
-        if (sscanf(line, "%llx %d %d %d %d %d %d %d %d %d %d", &k, &p, &b, &idx, &layout, &order, &bx,
-                   &by, &bz, &bs, &npt) == 11 &&
+        int p, b, layout, order, bx, by, bz, bs, npt, idx, waves;
+        if (sscanf(line, "%llx %d %d %d %d %d %d %d %d %d %d %d", &k, &p, &b, &idx, &layout, &order, &bx,
+                   &by, &bz, &bs, &npt, &waves) == 12 &&
             k == key && p == prec && b == bb) {
             best->index_kind = idx;
             best->layout = layout;
@@ -98,6 +98,7 @@ static bool cache_lookup(const char* path, uint64_t key, int prec, int bb, plb_c
             best->brick[2] = bz;
             best->block_size = bs;
             best->nodes_per_thread = npt;
+            best->waves_per_simd = waves;
             found = true;  // the last matching line wins
         }
     }
@@ -121,7 +122,8 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
         return PLB_OK;
 
     // Stage 1: index encoding x layout x node order at 256 threads, one node
-    // per thread.  Stage 2: block size and nodes per thread for the winner.
+    // per thread, no occupancy cap.  Stage 2: block size x nodes per thread x
+    // waves per SIMD for the winner.
     // Every index encoding gives the same sources, so the choice changes speed
     // only, never results.
     double best_mlups = -1.0;
@@ -168,28 +170,54 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
                 cfg.block_size = 256;
                 cfg.nodes_per_thread = 1;
                 cfg.graph_steps = 0;
+                cfg.waves_per_simd = 0;
                 double m = 0.0;
                 rc = time_cfg(cfg, &m);
                 if (rc) return rc;
             }
+    // Stage 2 changes only the launch (block size, nodes per thread, waves
+    // per SIMD), so every candidate runs on one context: the comparison is
+    // free of the allocation-placement spread between contexts.
     const plb_config stage1 = *best;
+    plb_ctx* c = nullptr;
+    rc = plb_create_from_flag(&c, &stage1, flag, nx, ny, nz, num_p);
+    if (rc) return rc;
+    const double n_nodes = (double)plb_num_nodes(c);
+    int64_t it = 0;
     for (int bs : {128, 256, 512})
-        for (int npt : {1, 2}) {
-            plb_config cfg = stage1;
-            cfg.block_size = bs;
-            cfg.nodes_per_thread = npt;
-            double m = 0.0;
-            rc = time_cfg(cfg, &m);
-            if (rc) return rc;
-        }
+        for (int npt : {1, 2})
+            for (int waves : {0, 6, 5, 4}) {
+                plb_config cfg = stage1;
+                cfg.block_size = bs;
+                cfg.nodes_per_thread = npt;
+                cfg.waves_per_simd = waves;
+                c->cfg = cfg;
+                double ms = 0.0;
+                const int warm = std::max(2, steps / 4);
+                rc = plb_step(c, it, warm);
+                if (!rc) rc = plb_step_timed(c, it + warm, steps, &ms);
+                it += warm + steps;
+                if (rc) {
+                    plb_destroy(c);
+                    return rc;
+                }
+                const double m = n_nodes * steps / (ms * 1e-3) / 1e6;
+                if (results && nres < max_results) results[nres] = {cfg, m, ms};
+                nres++;
+                if (m > best_mlups) {
+                    best_mlups = m;
+                    *best = cfg;
+                }
+            }
+    plb_destroy(c);
     if (n_results) *n_results = std::min(nres, max_results);
     if (cache_path) {
         FILE* fp = fopen(cache_path, "a");
         if (fp) {
-            fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
+            fprintf(fp, "%016llx %d %d %d %d %d %d %d %d %d %d %d %.1f\n", (unsigned long long)key,
                     base->precision, base->bb_mode, best->index_kind, best->layout, best->ordering,
                     best->brick[0], best->brick[1], best->brick[2], best->block_size,
-                    best->nodes_per_thread, best_mlups);
+                    best->nodes_per_thread, best->waves_per_simd, best_mlups);
             fclose(fp);
         }
     }

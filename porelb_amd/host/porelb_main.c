@@ -19,10 +19,12 @@
  *   PLB_LAYOUT soa|aosoa (aosoa)  PLB_ORDER reference|brick|pencil (pencil)
  *   PLB_BRICK bx,by,bz (pencil: nx,64,1)  PLB_INDEX i32|tile8|bitmap (tile8)
  *   PLB_BLOCK (128 below 4 M pore nodes, else 256)  PLB_GRAPH (0: steps per graph replay)
+ *   PLB_WAVES (step-kernel waves per SIMD, 0 = no cap; 5 for FP64/FP32 from
+ *   4 M pore nodes, else 0)
  *   PLB_DEVICE (0)  PLB_PRINT 1|0 (1: the per-step line of porelb.c:435)
  *   PLB_AUTOTUNE cachefile: time index encoding x layout x node order, then
- *   block size x nodes per thread, on this V.bin once; reuse the result from
- *   cachefile after
+ *   block size x nodes per thread x waves per SIMD, on this V.bin once; reuse
+ *   the result from cachefile after
  *   PLB_ASYNC_SAVE 1|0 (1): part files are written by a worker thread from one
  *   of two host buffers while the GPU keeps stepping (same files)
  *   PLB_CHECKPOINT file: populations written after the last step
@@ -188,6 +190,10 @@ int main(int argc, char *argv[])
     cfg.index_kind = env_choice("PLB_INDEX", "i32", "tile8", "bitmap", PLB_INDEX_TILE8);
     cfg.block_size = env_int("PLB_BLOCK", v.num_node < (1 << 22) ? 128 : 256);
     cfg.graph_steps = env_int("PLB_GRAPH", cfg.graph_steps);
+    /* At most 5 step-kernel waves per SIMD on large FP64/FP32 lattices (measured
+     * faster there; FP16 runs best uncapped). */
+    cfg.waves_per_simd =
+        env_int("PLB_WAVES", cfg.precision != PLB_FP16 && v.num_node >= (1 << 22) ? 5 : 0);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
     const int print = env_int("PLB_PRINT", 1);
@@ -218,9 +224,10 @@ int main(int argc, char *argv[])
             exit(rc);
         }
         cfg = best;
-        printf(">>> autotune: index %d layout %d order %d brick %d,%d,%d block %d nodes/thread %d\n",
+        printf(">>> autotune: index %d layout %d order %d brick %d,%d,%d block %d nodes/thread %d "
+               "waves/SIMD %d\n",
                cfg.index_kind, cfg.layout, cfg.ordering, cfg.brick[0], cfg.brick[1], cfg.brick[2],
-               cfg.block_size, cfg.nodes_per_thread);
+               cfg.block_size, cfg.nodes_per_thread, cfg.waves_per_simd);
     }
 
     plb_ctx *c = NULL;

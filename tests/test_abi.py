@@ -37,7 +37,8 @@ def test_invalid_config_rejected_before_device(plb):
     L = plb.lib()
     h = C.c_void_p()
     flag = (C.c_uint8 * 8)(*([1] * 8))
-    for field, value in [("tau", 0.0), ("precision", 7), ("block_size", 100), ("layout", 3)]:
+    for field, value in [("tau", 0.0), ("precision", 7), ("block_size", 100), ("layout", 3),
+                         ("waves_per_simd", 9), ("waves_per_simd", -1), ("graph_steps", 3)]:
         cfg = plb.default_config(**{field: value})
         rc = L.plb_create_from_flag(C.byref(h), C.byref(cfg), flag, 2, 2, 2, 1)
         assert rc == -20, field

@@ -13,14 +13,15 @@ def masks(plb):
             plb.mask_spheres((17, 23, 19), 2.0, 4.0, 0.35, 3)]
 
 
-@pytest.mark.parametrize("prec,index,order,bb", [(0, 0, 0, 1), (1, 0, 0, 1), (1, 1, 0, 1), (2, 1, 0, 1),
-                                                 (1, 1, 2, 1), (0, 0, 1, 1), (0, 1, 2, 0), (1, 1, 2, 0)])
-def test_batch_equals_single_runs(plb, prec, index, order, bb):
+@pytest.mark.parametrize("prec,index,order,bb,waves", [(0, 0, 0, 1, 0), (1, 0, 0, 1, 0), (1, 1, 0, 1, 0),
+                                                       (2, 1, 0, 1, 0), (1, 1, 2, 1, 0), (0, 0, 1, 1, 0),
+                                                       (0, 1, 2, 0, 0), (1, 1, 2, 0, 0), (1, 1, 2, 1, 5)])
+def test_batch_equals_single_runs(plb, prec, index, order, bb, waves):
     fl = masks(plb)
     taus, gxs = [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6]
     brick = (8, 4, 2) if order == 1 else (64, 4, 1)
     b = plb.Batch(fl, taus, gxs, precision=prec, bb_mode=bb, index_kind=index, layout=index,
-                  fp16_scale_log2=20, ordering=order, brick=brick)
+                  fp16_scale_log2=20, ordering=order, brick=brick, waves_per_simd=waves)
     steps, upd, ms = b.run(300, 100, 0.0)   # tol 0: nobody stops early
     assert list(steps) == [300, 300, 300]
     assert upd == 300 * sum(b.nodes)

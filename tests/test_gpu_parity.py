@@ -143,7 +143,10 @@ def test_fp64_layouts_orders_blocks_identical(plb, mode):
                 dict(index_kind=1, layout=1, ordering=2, brick=(44, 4, 1)),
                 dict(ordering=2, brick=(1000, 7, 3)),
                 dict(nodes_per_thread=2), dict(nodes_per_thread=2, index_kind=1, layout=1, block_size=128),
-                dict(nodes_per_thread=2, index_kind=1, layout=1, ordering=2, brick=(44, 8, 1), block_size=512)]
+                dict(nodes_per_thread=2, index_kind=1, layout=1, ordering=2, brick=(44, 8, 1), block_size=512),
+                dict(waves_per_simd=4), dict(index_kind=1, layout=1, ordering=2, brick=(44, 4, 1), waves_per_simd=5),
+                dict(waves_per_simd=3, block_size=128, nodes_per_thread=2), dict(waves_per_simd=1, block_size=1024),
+                dict(waves_per_simd=7, block_size=64, index_kind=1, layout=1)]
     for kw in variants:
         say("variant", mode, kw)
         with plb.Lattice.from_flag(flag, 2, tau=0.7, gx=1e-4, bb_mode=mode, **kw) as lat:

@@ -1,6 +1,7 @@
 """Seeded sweep over small random masks and random valid configurations: the
 FP64 path (every index encoding, layout, node order, brick, block size,
-nodes per thread and graph replay length, on 1-4 z-slab ranks) must equal
+nodes per thread, graph replay length and waves-per-SIMD cap, on 1-4 z-slab
+ranks) must equal
 the oracle's pull restatement bit for bit, populations and macros.  Dims go
 down to 1 voxel and node counts are ragged against the 64-node tiles.  The
 same configurations with FP32 shifted storage stay within FP32 rounding of
@@ -42,6 +43,8 @@ def draw_case(k):
         kw["nodes_per_thread"] = 2
     if rng.random() < 0.3:
         kw["graph_steps"] = int(rng.choice([2, 4, 6]))
+    if rng.random() < 0.3:
+        kw["waves_per_simd"] = int(rng.integers(1, 9))
     tau, gx = float(rng.uniform(0.6, 1.4)), float(rng.uniform(-2e-4, 2e-4))
     return flag, np_, mode, kw, tau, gx
 

@@ -19,15 +19,17 @@ def test_autotune_cache_and_result(plb, tmp_path):
     cache = str(tmp_path / "tune.txt")
     base = plb.default_config(tau=0.8, gx=1e-5, precision=plb.FP64, bb_mode=plb.BB_REFCOMPAT)
     best, res = plb.autotune(flag, 1, 5, cache, cfg=base)
-    assert len(res) == 2 * 2 * 4 + 3 * 2 and all(r["mlups"] > 0 for r in res)
+    # stage 1: 2 index kinds x 2 layouts x 4 orders; stage 2: 3 blocks x 2 nodes/thread x 4 wave caps
+    assert len(res) == 2 * 2 * 4 + 3 * 2 * 4 and all(r["mlups"] > 0 for r in res)
     assert {r["index_kind"] for r in res} == {plb.INDEX_I32, plb.INDEX_TILE8}
+    assert {r["waves"] for r in res[16:]} == {0, 6, 5, 4}
     assert max(r["mlups"] for r in res) == pytest.approx(max(r["mlups"] for r in res))
     text = open(cache).read().split()
-    assert len(text) == 12
+    assert len(text) == 13
     t0 = time.perf_counter()
     best2, res2 = plb.autotune(flag, 1, 5, cache, cfg=base)
     assert res2 == [] and time.perf_counter() - t0 < 2.0
-    for k in ("index_kind", "layout", "ordering", "block_size", "nodes_per_thread"):
+    for k in ("index_kind", "layout", "ordering", "block_size", "nodes_per_thread", "waves_per_simd"):
         assert getattr(best, k) == getattr(best2, k)
     # the tuned shape changes speed only, never the FP64 result
     V, start = O.build_V(flag, 1)
