@@ -102,12 +102,13 @@ typedef struct {
     int graph_steps;     /* 0: one kernel launch per step; G > 0 (even, <= 4096):
                             runs of G steps replay one captured HIP graph
                             (single lattices only; bit-identical results)      */
-    int waves_per_simd;  /* 0: as many resident waves as registers allow;
-                            1..8: at most this many step-kernel waves per SIMD,
-                            enforced by reserving each block a share of the
-                            CU's LDS (fewer concurrent streams per CU; exact
-                            when 4*w*64/block_size is a whole number of blocks
-                            and the share is <= 64 KB; results unchanged)      */
+    int waves_per_simd;  /* 0 (or 8): as many resident waves as registers
+                            allow; 1..7: about this many step-kernel waves per
+                            SIMD.  Each block reserves share = LDS per CU /
+                            max(1, 4*w*64/block_size) bytes of LDS, rounded
+                            down to 512 and at most 64 KB, so a CU holds
+                            floor(LDS per CU / share) blocks (fewer concurrent
+                            streams per CU; results unchanged)                 */
 } plb_config;
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
