@@ -182,3 +182,7 @@ def test_porelb_bad_environment_values(tmp_path, plb):
                       ({"PLB_SAVE": "0"}, "PLB_SAVE must be >= 1")]:
         r = run([os.path.join(BIN, "porelb.x"), "10"], tmp_path, env=env)
         assert r.returncode != 0 and text in r.stderr, (env, r.stderr)
+    # values the library's configuration check rejects (before any device call)
+    for env in ({"PLB_WAVES": "9"}, {"PLB_WAVES": "-1"}, {"PLB_BLOCK": "100"}, {"PLB_GRAPH": "3"}):
+        r = run([os.path.join(BIN, "porelb.x"), "10"], tmp_path, env=env)
+        assert r.returncode == (-20) & 0xFF and "libplb: invalid argument" in r.stderr, (env, r.stderr)
