@@ -157,7 +157,7 @@ def study_precision(P, max_steps, check, tol):
     runs = {}
     for prec in (0, 1, 2):
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
-                               ordering=2, brick=w["brick"], block_size=256)
+                               ordering=2, brick=w["brick"], block_size=256, waves_per_simd=w["waves"])
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
             t0 = time.perf_counter()
@@ -193,14 +193,15 @@ def study_precision(P, max_steps, check, tol):
 def study_config4(P, steps, chunk=500):
     """Config 4: the 1024^3 sphere pack (porosity 0.20, tau 0.6, g_x 5e-7),
     the same number of steps in FP64 and in FP32 storage (both half-way
-    bounce-back, TILE8, pencils).  Reports MLUPS and the roofline fraction of
-    each run, and the FP32 errors in K and in the velocity field against FP64."""
+    bounce-back, TILE8, pencils, at most 5 waves per SIMD).  Reports MLUPS and
+    the roofline fraction of each run, and the FP32 errors in K and in the
+    velocity field against FP64."""
     w = WORKLOADS["spheres1024-fp32"]
     flag = make_mask(P, w)
     runs = {}
     for prec in (0, 1):
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
-                               ordering=2, brick=w["brick"], block_size=256)
+                               ordering=2, brick=w["brick"], block_size=256, waves_per_simd=w["waves"])
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
             ms, it = 0.0, 0
