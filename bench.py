@@ -329,12 +329,13 @@ def study_outputs(P, reps=5):
     print(json.dumps(out), flush=True)
 
 
-def study_batch(P, n_samples, max_steps, check, tol, seed=100):
+def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
     """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
     sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
     TILE8, tau 1.0, g_x 1e-6.  One batched launch per step over all active
     samples (per-sample convergence stop) vs the same samples run one after
-    another (one sample per launch, same kernels and convergence rule)."""
+    another (one sample per launch, same kernels and convergence rule); at
+    most `waves` step-kernel waves per SIMD in both."""
     rng = np.random.default_rng(seed)
     flags = []
     for k in range(n_samples):
@@ -345,7 +346,8 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100):
         else:
             flags.append(P.mask_noise((L, L, L), 2.0, phi, seed + k))
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
-    kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256)
+    kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256,
+              waves_per_simd=waves)
     b = P.Batch(flags, taus, gxs, **kw)
     steps, upd, ms = b.run(max_steps, check, tol)
     red = b.reduce()
@@ -359,7 +361,7 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100):
         seq_ms += m
         seq_steps.append(int(st[0]))
         one.close()
-    out = {"study": "batch", "samples": n_samples, "pore_nodes_total": int(sum(nodes)),
+    out = {"study": "batch", "samples": n_samples, "pore_nodes_total": int(sum(nodes)), "waves_per_simd": waves,
            "tol": tol, "check_every": check, "max_steps": max_steps,
            "batched": {"updates": int(upd), "ms": round(ms, 1), "mlups": round(upd / (ms * 1e-3) / 1e6, 1)},
            "sequential": {"updates": int(seq_upd), "ms": round(seq_ms, 1),
@@ -402,7 +404,8 @@ def main():
         study_precision(P, args.max_steps, args.check, args.tol if args.tol is not None else 1e-8)
         return
     if args.study == "batch":
-        study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6)
+        study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6,
+                    waves=args.waves if args.waves is not None else 5)
         return
     if args.study == "outputs":
         study_outputs(P)
