@@ -55,3 +55,29 @@ def test_no_gpu_fails_loudly(plb):
     with pytest.raises(plb.PlbError) as ei:
         plb.Lattice.from_flag(flag)
     assert ei.value.code in (-24, -21)
+
+
+def test_ctypes_structs_match_the_header(plb, tmp_path):
+    """The Python mirrors of the C structs have the header's size and field
+    offsets (compiled and printed by gcc from include/plb.h and include/plb_tune.h)."""
+    structs = {"plb_config": plb.Config, "plb_reduction": plb.Reduction, "plb_fault": plb.Fault,
+               "plb_sample": plb.Sample, "plb_tune_result": plb.TuneResult}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "plb.h"', '#include "plb_tune.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        name, field, val = line.split()
+        got[(name, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
