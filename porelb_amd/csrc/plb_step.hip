@@ -532,9 +532,9 @@ __global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const u
         for (int j = 1; j < Q; j++) {
             const int32_t desc = a.tdesc[tile * (Q - 1) + j - 1];
             if (desc >= 0) {
-                const int k = (j - 1) >> 2;
-                const uint32_t word = k < TILE8_FULL ? a.toff[(tile * TILE8_FULL + k) * 64 + lane]
-                                                     : (uint32_t)a.toff16[tile * 64 + lane];
+                const int wi = (j - 1) >> 2;  // packed word holding direction j
+                const uint32_t word = wi < TILE8_FULL ? a.toff[(tile * TILE8_FULL + wi) * 64 + lane]
+                                                      : (uint32_t)a.toff16[tile * 64 + lane];
                 const int32_t v = (int32_t)((word >> (8 * ((j - 1) & 3))) & 0xffu);
                 t[j] = v == 255 ? TAB_BOUNCE : (v == 254 ? TAB_CONST : desc + v);
             } else {
