@@ -244,6 +244,11 @@ int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double t
                   int64_t *steps_done, int64_t *updates, double *ms);
 /* Per-sample reductions after a run (n_samples entries). */
 int plb_batch_reduce(plb_batch *b, plb_reduction *out);
+/* rho, u_x, u_y, u_z of one sample in its V order (z,y,x lexicographic, one
+ * rank), plb_batch_num_nodes(b, sample) values each: the sample's last step
+ * of the last run (rho0 and 0 before any run) -- the arrays save_data writes
+ * (porelb.c:375-391), per sample.  Any pointer may be NULL. */
+int plb_batch_read_macros(plb_batch *b, int sample, double *rho, double *ux, double *uy, double *uz);
 int64_t plb_batch_num_nodes(const plb_batch *b, int sample);
 void plb_batch_destroy(plb_batch *b);
 

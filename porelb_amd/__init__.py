@@ -110,6 +110,7 @@ def _lib():
             "plb_batch_create": ([p, p, p, i32], i32),
             "plb_batch_run": ([p, i64, i64, d, p, p, p], i32),
             "plb_batch_reduce": ([p, p], i32),
+            "plb_batch_read_macros": ([p, i32, p, p, p, p], i32),
             "plb_batch_num_nodes": ([p, i32], i64),
             "plb_batch_destroy": ([p], None),
             "plb_mask_noise": ([p, i32, i32, i32, d, d, C.c_uint64], i32),
@@ -375,6 +376,13 @@ class Batch:
         arr = (Reduction * self.n)()
         _check(_lib().plb_batch_reduce(self._h, arr), "plb_batch_reduce")
         return [arr[s].as_dict() for s in range(self.n)]
+
+    def read_macros(self, sample):
+        """(4, n) rho, ux, uy, uz of one sample in its V order."""
+        out = np.zeros((4, self.nodes[sample]))
+        _check(_lib().plb_batch_read_macros(self._h, sample, _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                                            _ptr(out[3])), "plb_batch_read_macros")
+        return out
 
     def close(self):
         if self._h:

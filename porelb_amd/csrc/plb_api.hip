@@ -15,14 +15,6 @@
 
 #include "plb_internal.hpp"
 
-namespace plb {
-int launch_init_eq(plb_ctx* c);
-int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout);
-int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din);
-int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* dout);
-int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout);
-}  // namespace plb
-
 using namespace plb;
 
 void plb_set_hip_error(hipError_t e, const char* what, const char* file, int line)

@@ -8,7 +8,9 @@
 // nothing.  Every `check_every` steps the ux macros are stored and summed per
 // sample in FP64; a sample stops when |d<u_x>| / |<u_x>| < tol between two
 // checks.  A stopped sample's state stays in the ping-pong buffer that held
-// it at its last step (recorded per sample).
+// it at its last step (recorded per sample).  The macros are stored on every
+// check step, so after a run they hold each sample's last step; the batch
+// context's inv_perm maps each sample's V order into the concatenation.
 #include <math.h>
 #include <string.h>
 
@@ -53,12 +55,22 @@ extern "C" void plb_batch_destroy(plb_batch* b)
     delete b;
 }
 
+// inv[off + g] = off + (sample order of V-order node g), identity without a
+// sample order.
+__global__ void k_batch_inv(uint32_t* inv, const uint32_t* sample_inv, int64_t n, int64_t off)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    inv[off + g] = (uint32_t)(off + (sample_inv ? (int64_t)sample_inv[g] : g));
+}
+
 static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* smp)
 {
     plb_ctx* c = b->c;
     const int bs = cfg->block_size;
     std::vector<int32_t*> tabs(b->S, nullptr);
     std::vector<uint64_t*> xyzs(b->S, nullptr);
+    std::vector<uint32_t*> invs(b->S, nullptr);
     int rc = PLB_OK;
     int64_t total = 0;
     for (int s = 0; s < b->S && !rc; s++) {
@@ -68,18 +80,15 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
                                  c->stream, &tabs[s], &xyzs[s], &ns, &start);
         if (!rc && cfg->ordering != PLB_ORDER_REFERENCE && ns > 0) {
             // The sample's node order (bricks or pencils with the sample's own
-            // x extent for pencils).  Batch outputs are per-sample sums, so
-            // the permutation itself is not kept.
+            // x extent for pencils); its V-order map is kept for read-backs.
             plb_config scfg = *cfg;
             if (scfg.ordering == PLB_ORDER_PENCIL) scfg.brick[0] = smp[s].nx;
-            uint32_t* inv = nullptr;
             int32_t* t2 = nullptr;
-            rc = compute_order(scfg, smp[s].nx, smp[s].ny, smp[s].nz, ns, xyzs[s], c->stream, &inv,
+            rc = compute_order(scfg, smp[s].nx, smp[s].ny, smp[s].nz, ns, xyzs[s], c->stream, &invs[s],
                                nullptr, nullptr);
             if (!rc && hipMalloc(&t2, sizeof(int32_t) * (Q - 1) * ns) != hipSuccess) rc = PLB_ERR_NOMEM;
-            if (!rc) rc = permute_ref_table(tabs[s], ns, inv, t2, c->stream);
+            if (!rc) rc = permute_ref_table(tabs[s], ns, invs[s], t2, c->stream);
             if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PLB_ERR_HIP;
-            if (inv) (void)hipFree(inv);
             if (!rc) {
                 (void)hipFree(tabs[s]);
                 tabs[s] = t2;
@@ -98,6 +107,7 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         hipError_t e = hipMalloc(&c->tab, sizeof(int32_t) * (Q - 1) * total);
         if (e == hipSuccess) e = hipMemsetAsync(c->tab, 0, sizeof(int32_t) * (Q - 1) * total, c->stream);
         if (e == hipSuccess) e = hipMalloc(&c->node_xyz, sizeof(uint64_t) * total);
+        if (e == hipSuccess) e = hipMalloc(&c->inv_perm, sizeof(uint32_t) * total);
         if (e != hipSuccess) rc = e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
     }
     for (int s = 0; s < b->S && !rc; s++) {
@@ -106,11 +116,17 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
             hipMemcpyAsync(c->node_xyz + b->off[s], xyzs[s], sizeof(uint64_t) * b->n[s],
                            hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
             rc = PLB_ERR_HIP;
+        if (!rc && b->n[s] > 0) {
+            hipLaunchKernelGGL(k_batch_inv, dim3((unsigned)((b->n[s] + 255) / 256)), dim3(256), 0, c->stream,
+                               c->inv_perm, invs[s], b->n[s], b->off[s]);
+            if (hipGetLastError() != hipSuccess) rc = PLB_ERR_HIP;
+        }
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = PLB_ERR_HIP;
     for (int s = 0; s < b->S; s++) {
         if (tabs[s]) (void)hipFree(tabs[s]);
         if (xyzs[s]) (void)hipFree(xyzs[s]);
+        if (invs[s]) (void)hipFree(invs[s]);
     }
     c->index_bytes = (Q - 1) * 4.0;
     if (!rc && cfg->index_kind == PLB_INDEX_TILE8) rc = compress_tile8(c);
@@ -299,6 +315,34 @@ extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_ever
         for (int s = 0; s < b->S; s++) steps_done[s] = done[s];
     if (updates) *updates = upd;
     return PLB_OK;
+}
+
+extern "C" int plb_batch_read_macros(plb_batch* b, int sample, double* rho, double* ux, double* uy,
+                                     double* uz)
+{
+    if (!b || sample < 0 || sample >= b->S) return PLB_ERR_INVALID;
+    plb_ctx* c = b->c;
+    PLB_HIP(hipSetDevice(c->device));
+    const int64_t n = b->n[sample], off = b->off[sample];
+    if (n == 0) return PLB_OK;
+    double* outs[4] = {rho, ux, uy, uz};
+    const int64_t chunk = std::min<int64_t>(n, (int64_t)8 << 20);
+    double* d = nullptr;
+    PLB_HIP(hipMalloc(&d, sizeof(double) * chunk));
+    int rc = PLB_OK;
+    for (int comp = 0; comp < 4 && !rc; comp++) {
+        if (!outs[comp]) continue;
+        for (int64_t k = 0; k < n && !rc; k += chunk) {
+            const int64_t m = std::min(chunk, n - k);
+            rc = launch_read_macros(c, comp, off + k, m, d);
+            if (!rc && hipMemcpyAsync(outs[comp] + k, d, sizeof(double) * m, hipMemcpyDeviceToHost,
+                                      c->stream) != hipSuccess)
+                rc = PLB_ERR_HIP;
+            if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PLB_ERR_HIP;
+        }
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 extern "C" int plb_batch_reduce(plb_batch* b, plb_reduction* out)

@@ -203,6 +203,10 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
                       int n_active, const int32_t* block_sample, const int64_t* sample_end,
                       const double* omega, const double* gx);
 int launch_init_eq(plb_ctx* c);
+int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout);
+int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din);
+int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* dout);
+int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout);
 int launch_reduce_range(plb_ctx* c, int buf, int64_t first, int64_t n, bool with_mass,
                         double* out2_host);
 int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,

@@ -50,3 +50,5 @@ def test_batch_convergence_stops_per_sample(plb):
                                    bb_mode=plb.BB_HALFWAY) as lat:
             lat.step(0, int(steps[s]))
             assert lat.reduce()["sum_ux"] == red[s]["sum_ux"]
+            # the sample's macros are those of its own last step, in V order
+            assert np.array_equal(b.read_macros(s), lat.read_macros()), s

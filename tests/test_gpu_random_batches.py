@@ -1,9 +1,9 @@
 """Seeded sweep over random batches: 2-5 small random masks per batch, each
 with its own tau and g_x, in random valid batch configurations (bounce-back
 mode, I32 or TILE8, layout, node order, block size, waves-per-SIMD cap).
-Every sample's FP64 mass, sum u_x and K after the batched run equal the
-oracle's reduction of that sample's own pull run to 1e-12 relative (the GPU
-sums in node order, the oracle sequentially)."""
+Every sample's FP64 macros (read back in V order) equal the oracle's pull
+run bit for bit, and its mass, sum u_x and K equal the oracle's reduction to
+1e-12 relative (the GPU sums in node order, the oracle sequentially)."""
 import numpy as np
 import pytest
 
@@ -46,6 +46,7 @@ def test_random_batch_matches_oracle(plb, k):
         steps, upd, _ = b.run(STEPS, STEPS, 0.0)  # tol 0: every sample runs all steps
         red = b.reduce()
         nodes = list(b.nodes)
+        macs = [b.read_macros(s) for s in range(len(flags))]
     finally:
         b.close()
     assert list(steps) == [STEPS] * len(flags)
@@ -55,6 +56,7 @@ def test_random_batch_matches_oracle(plb, k):
         tab = O.pull_table(V, start, kw["bb_mode"])
         f0, _ = O.init_equilibrium(len(V))
         mac = O.pull_run(tab, f0, STEPS, taus[s], gxs[s])
+        assert np.array_equal(macs[s], mac), s  # V order, bit for bit
         want = O.reduce(f0, mac[1], f.shape, taus[s], gxs[s])
         for key in ("mass", "sum_ux", "K"):
             assert red[s][key] == pytest.approx(want[key], rel=1e-12, abs=1e-300), (s, key)
