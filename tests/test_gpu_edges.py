@@ -49,6 +49,28 @@ def test_degenerate_dims_bitwise(plb, dims, mode):
             assert np.array_equal(lat.read_macros(), mac), kw
 
 
+@pytest.mark.parametrize("np_", [8, 16])
+def test_one_plane_per_rank_reference_order(plb, np_):
+    """z-slabs one plane thick (nz = num_p): every z-link crosses ranks, so the
+    reference's receive-after-stream order (porelb.c:285-312) decides every
+    slot with a z component.  FP64 table, populations and macros equal the
+    oracle's emulated ranks bit for bit, both from the mask and from V.bin
+    records."""
+    rng = np.random.default_rng(np_)
+    flag = (rng.random((np_, 9, 11)) < 0.6).astype(np.uint8)
+    V, start, f0, mac = oracle_run(flag, np_, 25, 0.7, 2e-4)
+    assert len(start) == np_ and all(start[k] <= start[k + 1] for k in range(np_ - 1))
+    for kw in (dict(), dict(index_kind=1, layout=1, ordering=2, brick=(11, 4, 1), waves_per_simd=5)):
+        with plb.Lattice.from_flag(flag, np_, tau=0.7, gx=2e-4, **kw) as lat:
+            assert np.array_equal(lat.read_table(), O.pull_table(V, start, O.BB_REFCOMPAT)), kw
+            lat.step(0, 25)
+            assert np.array_equal(lat.read_populations(), f0), kw
+            assert np.array_equal(lat.read_macros(), mac), kw
+    with plb.Lattice.from_V(V, start, flag.shape, tau=0.7, gx=2e-4) as lat:
+        lat.step(0, 25)
+        assert np.array_equal(lat.read_macros(), mac)
+
+
 def test_single_pore_surrounded_by_solid(plb):
     flag = np.zeros((5, 5, 5), dtype=np.uint8)
     flag[2, 2, 2] = 1
