@@ -186,6 +186,15 @@ typedef struct {
     int64_t n_nodes;
 } plb_reduction;
 int plb_reduce(plb_ctx *c, plb_reduction *out);
+/* Steady-state run (new; the reference always runs it_max steps): steps from
+ * it0 in runs of check_every, and after each run compares <u_x> with the one
+ * check_every steps before; stops when |d<u_x>| <= tol * max(|<u_x>|,
+ * 1e-3 |gx|) (the rule of plb_batch_run) or after max_steps steps.
+ * *steps_done = steps taken, *out = the reductions of the last step (whose
+ * macros are stored).  tol = 0 runs all max_steps.  A density-guard fault
+ * returns PLB_ERR_DENSITY as plb_step does. */
+int plb_run_to_convergence(plb_ctx *c, int64_t it0, int64_t max_steps, int64_t check_every, double tol,
+                           int64_t *steps_done, plb_reduction *out);
 
 /* Density-guard record: first step with a violation, and at that step the
  * lowest V-order node, its rank pid and local id, coordinates and rho

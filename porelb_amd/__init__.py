@@ -110,6 +110,7 @@ def _lib():
             "plb_batch_create": ([p, p, p, i32], i32),
             "plb_batch_run": ([p, i64, i64, d, p, p, p], i32),
             "plb_batch_reduce": ([p, p], i32),
+            "plb_run_to_convergence": ([p, i64, i64, i64, d, p, p], i32),
             "plb_batch_read_macros": ([p, i32, p, p, p, p], i32),
             "plb_batch_num_nodes": ([p, i32], i64),
             "plb_batch_destroy": ([p], None),
@@ -302,6 +303,14 @@ class Lattice:
         r = Reduction()
         _check(_lib().plb_reduce(self._h, C.byref(r)), "plb_reduce")
         return r.as_dict()
+
+    def run_to_convergence(self, it0, max_steps, check_every, tol):
+        """plb_run_to_convergence: (steps taken, reductions of the last step)."""
+        n = C.c_int64(0)
+        r = Reduction()
+        _check(_lib().plb_run_to_convergence(self._h, it0, max_steps, check_every, tol, C.byref(n),
+                                             C.byref(r)), "plb_run_to_convergence")
+        return n.value, r.as_dict()
 
     def last_fault(self):
         f = Fault()

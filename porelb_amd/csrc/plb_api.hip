@@ -474,6 +474,32 @@ extern "C" int plb_reduce(plb_ctx* c, plb_reduction* out)
     return PLB_OK;
 }
 
+extern "C" int plb_run_to_convergence(plb_ctx* c, int64_t it0, int64_t max_steps, int64_t check_every,
+                                      double tol, int64_t* steps_done, plb_reduction* out)
+{
+    if (!c || !steps_done || !out || it0 < 0 || max_steps < 0 || check_every < 1 || !(tol >= 0.0))
+        return PLB_ERR_INVALID;
+    Range range("plb_run_to_convergence");
+    *steps_done = 0;
+    int rc = plb_reduce(c, out);  // the state before the first run (macros as stored)
+    if (rc) return rc;
+    double prev = NAN;
+    const double umin = 1e-3 * fabs(c->cfg.gx);
+    for (int64_t s = 0; s < max_steps;) {
+        const int64_t m = std::min(check_every, max_steps - s);
+        rc = plb_step(c, it0 + s, m);
+        if (rc) return rc;
+        s += m;
+        *steps_done = s;
+        rc = plb_reduce(c, out);
+        if (rc) return rc;
+        const double mean = out->mean_ux;
+        if (tol > 0.0 && !isnan(prev) && fabs(mean - prev) <= tol * std::max(fabs(mean), umin)) break;
+        prev = mean;
+    }
+    return PLB_OK;
+}
+
 extern "C" int plb_last_fault(plb_ctx* c, plb_fault* out)
 {
     if (!c || !out) return PLB_ERR_INVALID;

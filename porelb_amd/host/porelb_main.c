@@ -25,6 +25,9 @@
  *   PLB_AUTOTUNE cachefile: time index encoding x layout x node order, then
  *   block size x nodes per thread x waves per SIMD, on this V.bin once; reuse
  *   the result from cachefile after
+ *   PLB_TOL tol, PLB_CHECK m (off; 1000): stop before it_max once <u_x> changes by
+ *   at most tol * max(|<u_x>|, 1e-3 |gx|) between checks m steps apart (the
+ *   rule of plb_run_to_convergence); the final part files carry the step reached
  *   PLB_ASYNC_SAVE 1|0 (1): part files are written by a worker thread from one
  *   of two host buffers while the GPU keeps stepping (same files)
  *   PLB_CHECKPOINT file: populations written after the last step
@@ -32,6 +35,7 @@
  *   save_state/recovery_state stubs of porelb.c:367-373)
  */
 #define _POSIX_C_SOURCE 199309L
+#include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -201,6 +205,12 @@ int main(int argc, char *argv[])
         fprintf(stderr, ">>> PLB_SAVE must be >= 1\n");
         exit(-10);
     }
+    const double tol = env_dbl("PLB_TOL", 0.0);
+    const int check_every = env_int("PLB_CHECK", 1000);
+    if (!(tol >= 0.0) || check_every < 1) {
+        fprintf(stderr, ">>> PLB_TOL must be >= 0 and PLB_CHECK >= 1\n");
+        exit(-10);
+    }
 
     const char *tune = getenv("PLB_AUTOTUNE");
     if (tune && *tune) {
@@ -281,10 +291,15 @@ int main(int argc, char *argv[])
     int kbuf = 0;
     double t1 = wtime();
     int it = it0;
-    while (it < it_max) {
+    /* convergence checks (PLB_TOL > 0) every check_every steps from it0 */
+    long long next_check = tol > 0.0 ? (long long)it0 + check_every : -1;
+    double prev_ux = NAN;
+    int converged = 0;
+    while (it < it_max && !converged) {
         /* run up to and including the next save step (or to the end) */
         int next_save = ((it + save_every - 1) / save_every) * save_every;
         int last = next_save < it_max ? next_save : it_max - 1;
+        if (next_check > 0 && next_check - 1 < last) last = (int)(next_check - 1);
         rc = plb_step(c, it, (int64_t)last - it + 1);
         if (rc == PLB_ERR_DENSITY) {
             plb_fault ft;
@@ -307,13 +322,30 @@ int main(int argc, char *argv[])
             for (int s = it; s <= last; s++) printf(">>> Step : %20d \n", s);
         if (last % save_every == 0) save_all(c, sv, &kbuf, async, last);
         it = last + 1;
+        if (next_check > 0 && it == next_check) {
+            plb_reduction r;
+            rc = plb_reduce(c, &r);
+            if (rc) {
+                fprintf(stderr, ">>> libplb: %s\n", plb_strerror(rc));
+                saver_wait(&sv[0]);
+                saver_wait(&sv[1]);
+                exit(rc);
+            }
+            const double scale = fabs(r.mean_ux) > 1e-3 * fabs(cfg.gx) ? fabs(r.mean_ux) : 1e-3 * fabs(cfg.gx);
+            if (!isnan(prev_ux) && fabs(r.mean_ux - prev_ux) <= tol * scale) {
+                converged = 1;
+                printf(">>> Converged at step %d: <u_x> = %.10e, K = %.10e\n", it, r.mean_ux, r.K);
+            }
+            prev_ux = r.mean_ux;
+            next_check += check_every;
+        }
     }
     double t2 = wtime();
     printf("Elapsed time is %f\n", t2 - t1);
-    if (t2 > t1 && it_max > it0)
+    if (t2 > t1 && it > it0)
         printf("MLUPS %.3f (%lld pore nodes x %d steps)\n",
-               (double)N * (it_max - it0) / (t2 - t1) / 1e6, (long long)N, it_max - it0);
-    save_all(c, sv, &kbuf, async, it_max);
+               (double)N * (it - it0) / (t2 - t1) / 1e6, (long long)N, it - it0);
+    save_all(c, sv, &kbuf, async, it);
     saver_wait(&sv[0]);
     saver_wait(&sv[1]);
 
@@ -322,7 +354,7 @@ int main(int argc, char *argv[])
         double *f = (double *)plb_host_alloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
         FILE *fp = fopen(ckpt, "wb");
         uint32_t magic = CKPT_MAGIC;
-        int64_t hdr[2] = {it_max > it0 ? it_max : it0, N};
+        int64_t hdr[2] = {it, N};  /* the step reached: it_max, or earlier on convergence */
         if (!f || !fp || plb_read_populations(c, f)) {
             fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
             exit(-3);

@@ -179,7 +179,9 @@ def test_porelb_bad_environment_values(tmp_path, plb):
     for env, text in [({"PLB_PRECISION": "fp8"}, "bad value PLB_PRECISION=fp8"),
                       ({"PLB_INDEX": "tile4"}, "bad value PLB_INDEX=tile4"),
                       ({"PLB_BRICK": "4,4"}, "bad value PLB_BRICK=4,4"),
-                      ({"PLB_SAVE": "0"}, "PLB_SAVE must be >= 1")]:
+                      ({"PLB_SAVE": "0"}, "PLB_SAVE must be >= 1"),
+                      ({"PLB_TOL": "-1"}, "PLB_TOL must be >= 0"),
+                      ({"PLB_TOL": "1e-6", "PLB_CHECK": "0"}, "PLB_CHECK >= 1")]:
         r = run([os.path.join(BIN, "porelb.x"), "10"], tmp_path, env=env)
         assert r.returncode != 0 and text in r.stderr, (env, r.stderr)
     # values the library's configuration check rejects (before any device call)

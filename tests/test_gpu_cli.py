@@ -214,3 +214,28 @@ def test_porelb_part_file_write_failure(tmp_path, mode):
     (tmp_path / "part_000_0000010.bin").unlink()
     r = run([os.path.join(BIN, "porelb.x"), "20"], tmp_path, dict(env, PLB_CHECKPOINT="/dev/full"))
     assert r.returncode == (-3) & 0xFF and "cannot write checkpoint" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_porelb_stops_on_convergence(tmp_path):
+    """PLB_TOL / PLB_CHECK: porelb.x stops at the first converged check and its
+    final part files equal those of a plain run of that many steps."""
+    flag = (np.random.default_rng(33).random((12, 10, 14)) < 0.65).astype(np.uint8)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-5", "PLB_PRINT": "0", "PLB_BB": "halfway", "PLB_SAVE": "100000"}
+    wd = tmp_path / "tol"
+    wd.mkdir()
+    os.link(tmp_path / "V.bin", wd / "V.bin")
+    r = run([os.path.join(BIN, "porelb.x"), "50000"], wd, dict(env, PLB_TOL="1e-6", PLB_CHECK="40"))
+    assert r.returncode == 0, r.stderr
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith(">>> Converged at step")]
+    assert len(line) == 1, r.stdout[-500:]
+    steps = int(line[0].split()[4].rstrip(":"))
+    assert 0 < steps < 50000 and steps % 40 == 0
+    wd2 = tmp_path / "plain"
+    wd2.mkdir()
+    os.link(tmp_path / "V.bin", wd2 / "V.bin")
+    r = run([os.path.join(BIN, "porelb.x"), str(steps)], wd2, env)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(read_parts(wd, 2, steps), read_parts(wd2, 2, steps))

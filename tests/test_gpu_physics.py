@@ -87,3 +87,29 @@ def test_collision_conserves_mass_and_momentum(plb, prec, tol_mass, tol_mom, tau
     say("moments before", m0, "after", m1)
     assert abs(m1[0] / m0[0] - 1) < tol_mass
     assert np.all(np.abs(m1[1:] - m0[1:]) < tol_mom)
+
+
+def test_run_to_convergence_rule_and_state(plb):
+    """plb_run_to_convergence stops at the first check where <u_x> moved by at
+    most tol * max(|<u_x>|, 1e-3 g_x) since the previous check, and its state
+    equals plain stepping to the same step bit for bit; tol = 0 runs all steps."""
+    flag = plb.mask_spheres((24, 20, 28), 3.0, 5.0, 0.4, 11)
+    kw = dict(tau=0.9, gx=1e-5, bb_mode=plb.BB_HALFWAY, index_kind=1, layout=1)
+    tol, check = 1e-5, 50
+    with plb.Lattice.from_flag(flag, **kw) as lat:
+        n, r = lat.run_to_convergence(0, 20000, check, tol)
+        mac = lat.read_macros()
+    say("converged after", n, "steps, <u_x>", r["mean_ux"])
+    assert 0 < n < 20000 and n % check == 0
+    means = []
+    with plb.Lattice.from_flag(flag, **kw) as lat:
+        for s in range(0, n, check):
+            lat.step(s, check)
+            means.append(lat.reduce()["mean_ux"])
+        assert np.array_equal(lat.read_macros(), mac)
+        assert lat.reduce() == r
+    rule = [abs(b - a) <= tol * max(abs(b), 1e-3 * kw["gx"]) for a, b in zip(means, means[1:])]
+    assert rule[-1] and not any(rule[:-1])
+    with plb.Lattice.from_flag(flag, **kw) as lat:
+        n0, _ = lat.run_to_convergence(0, 130, check, 0.0)
+        assert n0 == 130
