@@ -188,3 +188,19 @@ def test_porelb_bad_environment_values(tmp_path, plb):
     for env in ({"PLB_WAVES": "9"}, {"PLB_WAVES": "-1"}, {"PLB_BLOCK": "100"}, {"PLB_GRAPH": "3"}):
         r = run([os.path.join(BIN, "porelb.x"), "10"], tmp_path, env=env)
         assert r.returncode == (-20) & 0xFF and "libplb: invalid argument" in r.stderr, (env, r.stderr)
+
+
+def test_bench_workloads_are_complete():
+    """Every bench workload carries the keys the benchmark line reads (checked
+    without a GPU, so a missing key fails here rather than on the GPU)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(REPO, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    need = {"kind", "dims", "tau", "gx", "precision", "bb", "index", "layout", "ordering", "brick", "block",
+            "label", "seed", "porosity"}
+    for name, w in bench.WORKLOADS.items():
+        assert need <= set(w), (name, need - set(w))
+        assert w["ordering"] != 2 or w["brick"][0] == w["dims"][2], name  # pencils span whole rows
+    assert set(bench.FP64_PLAIN) >= {"precision", "bb", "index", "layout", "ordering", "brick", "block", "waves"}
