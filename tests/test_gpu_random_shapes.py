@@ -81,3 +81,25 @@ def test_random_shape_fp32_within_rounding(plb, k):
     say("max |d rho|", np.abs(got[0] - mac[0]).max(), "max |d u|", np.abs(got[1:] - mac[1:]).max(), "max |u|", umax)
     assert np.abs(got[0] - mac[0]).max() <= 5e-7
     assert np.abs(got[1:] - mac[1:]).max() <= 1e-5 * umax + 1e-9
+
+
+@pytest.mark.parametrize("k", range(2, N_CASES, 4))
+def test_random_shape_fp16_within_rounding(plb, k):
+    """FP16 storage (deviations scaled by 2^k, FP32 collision) on the same
+    random configurations stays within FP16 rounding of the FP64 oracle after
+    17 steps: |d rho| <= 4e-6, |d u| <= 1e-2 max|u| + 1e-9 (observed: 4e-7 and
+    8e-4 max|u|)."""
+    flag, np_, mode, kw, tau, gx = draw_case(k)
+    say(k, "fp16", flag.shape[::-1], "np", np_, "bb", mode, kw)
+    V, start = O.build_V(flag, np_)
+    tab = O.pull_table(V, start, mode)
+    f0, _ = O.init_equilibrium(len(V))
+    mac = O.pull_run(tab, f0, 17, tau, gx)
+    with plb.Lattice.from_flag(flag, np_, tau=tau, gx=gx, bb_mode=mode, precision=plb.FP16, **kw) as lat:
+        lat.step(0, 17)
+        got = lat.read_macros()
+    umax = np.abs(mac[1:]).max()
+    say("fp16 max |d rho|", np.abs(got[0] - mac[0]).max(), "max |d u|", np.abs(got[1:] - mac[1:]).max(),
+        "max |u|", umax)
+    assert np.abs(got[0] - mac[0]).max() <= 4e-6
+    assert np.abs(got[1:] - mac[1:]).max() <= 1e-2 * umax + 1e-9
