@@ -113,7 +113,9 @@ def cpu_baseline(P, w, max_seconds=20.0):
     timer, on a 256^3 domain of the same kind and porosity."""
     import oracle as O
 
-    cores = min(16, os.cpu_count() or 1)
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    host = {"os_cpu_count": os.cpu_count(), "affinity": affinity}
+    cores = min(16, affinity)
     ranks = 1
     while ranks * 2 <= cores:
         ranks *= 2
@@ -130,7 +132,8 @@ def cpu_baseline(P, w, max_seconds=20.0):
                             workdir=wd, dump=False, timeout=900)
         subprocess.run(["rm", "-rf", wd])
         t = r["timing"]
-        return {"value": round(t["mlups"], 3), "unit": "pore-node MLUPS", "cores": ranks, "kind": "reference",
+        return {"value": round(t["mlups"], 3), "unit": "pore-node MLUPS", "cores": ranks, "ranks": ranks,
+                "host_cores": host, "kind": "reference",
                 "sample": f"{dims[0]}^3 {w['kind']} domain, porosity {w['porosity']}, {npore} pore nodes, "
                           f"{steps} FP64 steps, mpirun -np {ranks} (reference porelb.c, MPI_Wtime loop "
                           f"timer incl. per-step print and it%1000 save)"}
@@ -142,8 +145,8 @@ def cpu_baseline(P, w, max_seconds=20.0):
     t0 = time.perf_counter()
     O.pull_run(tab, f0, steps, w["tau"], w["gx"])
     dt = time.perf_counter() - t0
-    return {"value": round(len(V) * steps / dt / 1e6, 3), "unit": "pore-node MLUPS", "cores": 1,
-            "kind": "port", "sample": f"{dims[0]}^3 domain, {len(V)} pore nodes, {steps} FP64 steps, 1 thread"}
+    return {"value": round(len(V) * steps / dt / 1e6, 3), "unit": "pore-node MLUPS", "cores": 1, "ranks": 1,
+            "host_cores": host, "kind": "port", "sample": f"{dims[0]}^3 domain, {len(V)} pore nodes, {steps} FP64 steps, 1 thread"}
 
 
 def study_precision(P, max_steps, check, tol):
@@ -191,18 +194,18 @@ def study_precision(P, max_steps, check, tol):
     print(json.dumps(out), flush=True)
 
 
-def study_config4(P, steps, chunk=500):
-    """Config 4: the 1024^3 sphere pack (porosity 0.20, tau 0.6, g_x 5e-7),
-    the same number of steps in FP64 and in FP32 storage (both half-way
-    bounce-back, TILE8, pencils, at most 5 waves per SIMD).  Reports MLUPS and
-    the roofline fraction of each run, and the FP32 errors in K and in the
-    velocity field against FP64."""
-    w = WORKLOADS["spheres1024-fp32"]
-    flag = make_mask(P, w)
+def precision_runs(P, flag, w, precs, steps, chunk=500):
+    """The same number of steps from rest in each storage precision of
+    `precs` (the first one is the FP64 reference), all with half-way
+    bounce-back on the workload's TILE8 / pencil shape.  Returns per precision
+    the FP64 reductions after the last step, the mass drift, the velocity
+    field (V order) and the MLUPS of the timed steps."""
     runs = {}
-    for prec in (0, 1):
+    for prec in precs:
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
-                               ordering=2, brick=w["brick"], block_size=256, waves_per_simd=w["waves"])
+                               ordering=2, brick=w["brick"], block_size=256,
+                               waves_per_simd=0 if prec == 2 else w.get("waves", 5),
+                               fp16_rounding=w.get("fp16_rounding", 0))
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
             ms, it = 0.0, 0
@@ -210,24 +213,37 @@ def study_config4(P, steps, chunk=500):
                 n = min(chunk, steps - it)
                 ms += lat.step_timed(it, n)
                 it += n
-                log(f"config4 precision {prec}: step {it}")
+                log(f"precision {prec}: step {it}")
             r = lat.reduce()
-            mac = lat.read_macros()
             mlups = lat.N * steps / (ms * 1e-3) / 1e6
             runs[prec] = dict(K=r["K"], mean_ux=r["mean_ux"], mass_drift=(r["mass"] - m0) / m0, mlups=mlups,
                               roofline_frac=mlups * 1e6 * (2 * 19 * STORAGE_BYTES[prec] + 19 * 4) / (HBM_PEAK_GBS * 1e9),
-                              u=mac[1:].copy(), nodes=lat.N)
-        log(f"config4 precision {prec}: {mlups:.0f} MLUPS, K {r['K']:.9e}")
+                              u=lat.read_macros()[1:].copy(), nodes=lat.N)
+        log(f"precision {prec}: {mlups:.0f} MLUPS, K {r['K']:.9e}")
+    return runs
+
+
+def errors_vs_fp64(runs, prec):
     u64 = runs[0]["u"]
+    return {"K_rel_err": abs(runs[prec]["K"] / runs[0]["K"] - 1.0),
+            "velocity_rel_l2_err": float(np.linalg.norm(runs[prec]["u"] - u64) / np.linalg.norm(u64))}
+
+
+def study_config4(P, steps):
+    """Config 4: the 1024^3 sphere pack (porosity 0.20, tau 0.6, g_x 5e-7),
+    the same number of steps in FP64 and in FP32 storage (both half-way
+    bounce-back, TILE8, pencils, at most 5 waves per SIMD).  Reports MLUPS and
+    the roofline fraction of each run, and the FP32 errors in K and in the
+    velocity field against FP64."""
+    w = WORKLOADS["spheres1024-fp32"]
+    runs = precision_runs(P, make_mask(P, w), w, (0, 1), steps)
     out = {"study": "config4", "workload": w["label"], "pore_nodes": runs[0]["nodes"], "steps": steps,
            "bounce_back": "halfway", "results": {}}
     for prec, name in ((0, "fp64"), (1, "fp32_store")):
         r = runs[prec]
-        out["results"][name] = {
+        out["results"][name] = dict({
             "mlups": round(r["mlups"], 1), "roofline_frac": round(r["roofline_frac"], 4), "K": r["K"],
-            "mean_ux": r["mean_ux"], "mass_rel_drift": r["mass_drift"],
-            "velocity_rel_l2_err": float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64)),
-            "K_rel_err": abs(r["K"] / runs[0]["K"] - 1.0)}
+            "mean_ux": r["mean_ux"], "mass_rel_drift": r["mass_drift"]}, **errors_vs_fp64(runs, prec))
     print(json.dumps(out), flush=True)
 
 
@@ -330,13 +346,10 @@ def study_outputs(P, reps=5):
     print(json.dumps(out), flush=True)
 
 
-def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
-    """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
-    sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
-    TILE8, tau 1.0, g_x 1e-6.  One batched launch per step over all active
-    samples (per-sample convergence stop) vs the same samples run one after
-    another (one sample per launch, same kernels and convergence rule); at
-    most `waves` step-kernel waves per SIMD in both."""
+def batch_samples(P, n_samples=64, seed=100):
+    """The batching workload: n samples with edge L ~ U{64..192} and
+    porosity ~ U[0.15, 0.45], alternately sphere packs (r U[6,12]) and
+    smoothed noise (sigma 2), seeded; every sample runs at tau 1.0, g_x 1e-6."""
     rng = np.random.default_rng(seed)
     flags = []
     for k in range(n_samples):
@@ -346,6 +359,17 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
             flags.append(P.mask_spheres((L, L, L), 6.0, 12.0, phi, seed + k))
         else:
             flags.append(P.mask_noise((L, L, L), 2.0, phi, seed + k))
+    return flags
+
+
+def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
+    """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
+    sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
+    TILE8, tau 1.0, g_x 1e-6.  One batched launch per step over all active
+    samples (per-sample convergence stop) vs the same samples run one after
+    another (one sample per launch, same kernels and convergence rule); at
+    most `waves` step-kernel waves per SIMD in both."""
+    flags = batch_samples(P, n_samples, seed)
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
     kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256,
               waves_per_simd=waves)
@@ -393,6 +417,8 @@ def main():
     ap.add_argument("--check", type=int, default=200)
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--gate-steps", type=int, default=5000,
+                    help="steps of the FP64 vs reduced-precision comparison in the line (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is not supported "
@@ -448,12 +474,19 @@ def main():
     bpu = main_run["bytes_per_update"]
     achieved = algo_bpu * n / (avg_ms * 1e-3) / 1e9
     layout_gbs = bpu * n / (avg_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(REPO, "profiles", f"step_{args.workload}_pmc.md")
-    if os.path.exists(tfile):   # written from two rocprofv3 --pmc passes
+    # HBM bytes per launch from the committed rocprofv3 --pmc summary of this
+    # workload (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of
+    # known size); counters cannot be read inside this process, so the
+    # figure and the commit it was measured on are named in the line.
+    traffic, traffic_commit = None, None
+    tname = f"profiles/step_{args.workload}_pmc.md"
+    tfile = os.path.join(REPO, tname)
+    if os.path.exists(tfile):
         for line in open(tfile):
             if line.startswith("hbm_bytes_per_launch"):
                 traffic = float(line.split("=")[1])
+            if line.startswith("measured_at_commit"):
+                traffic_commit = line.split("=")[1].strip()
 
     secondary = {}
     if not args.skip_fp64 and w["precision"] != 0:
@@ -470,6 +503,24 @@ def main():
                      "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads, "
                                          "at most 5 waves per SIMD"}
 
+    # The metric's precision gate: the same steps from rest in FP64 and in the
+    # workload's storage (both half-way bounce-back), K = nu q / g_x
+    # (nu = (tau - 1/2)/3, porelb.c:89) and the velocity field compared.
+    precision = None
+    if args.gate_steps > 0 and w["precision"] != 0:
+        t0 = time.perf_counter()
+        runs = precision_runs(P, flag, w, (0, w["precision"]), args.gate_steps)
+        err = errors_vs_fp64(runs, w["precision"])
+        precision = {
+            "compare": f"{PREC_NAME[w['precision']]} vs f64, both half-way bounce-back, {args.gate_steps} steps "
+                       "from rest on this workload",
+            "K_f64": runs[0]["K"], "K": runs[w["precision"]]["K"],
+            "K_rel_err": err["K_rel_err"], "gate_K_rel_err": 1e-4, "pass": err["K_rel_err"] <= 1e-4,
+            "velocity_rel_l2_err": err["velocity_rel_l2_err"],
+            "mass_rel_drift_f64": runs[0]["mass_drift"], "mass_rel_drift": runs[w["precision"]]["mass_drift"],
+            "wall_s": round(time.perf_counter() - t0, 1)}
+        del runs
+
     out = {
         "metric": "pore-node MLUPS",
         "value": round(mlups, 1),
@@ -481,7 +532,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if w["precision"] == 0 else "f32",
+        "dtype": {0: "f64", 1: "f32", 2: "f16-store/f32"}[w["precision"]],
         "data": "synthetic (seeded periodic mask)",
         "config": {"workload": w["label"], "pore_nodes": n, "storage": PREC_NAME[w["precision"]],
                    "bounce_back": "halfway" if w["bb"] else "reference order",
@@ -492,15 +543,19 @@ def main():
                    "waves_per_simd": shape["waves"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": (f"{tname}, rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, measured at commit "
+                                        f"{traffic_commit}" if traffic else None),
                      "algorithmic_bytes_per_update": algo_bpu, "kernel_ms": round(avg_ms, 4),
                      "layout_bytes_per_update": round(bpu, 3), "layout_gbs": round(layout_gbs, 1),
-                     "measured_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac_of_peak": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                              if traffic else None),
                      "copy_ceiling_gbs": round(copy_gbs, 1),
-                     "measured_frac_of_copy": (round(traffic / (avg_ms * 1e-3) / 1e9 / copy_gbs, 4)
-                                               if traffic else None)},
+                     "traffic_frac_of_copy": (round(traffic / (avg_ms * 1e-3) / 1e9 / copy_gbs, 4)
+                                              if traffic else None)},
         "cpu_baseline": cpu,
         "secondary": secondary,
-        "K": main_run["reduction"]["K"],
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
 

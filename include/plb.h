@@ -109,7 +109,13 @@ typedef struct {
                             down to 512 and at most 64 KB, so a CU holds
                             floor(LDS per CU / share) blocks (fewer concurrent
                             streams per CU; results unchanged)                 */
+    int fp16_rounding;   /* FP16 storage: PLB_ROUND_NEAREST (round to nearest
+                            even) or PLB_ROUND_STOCHASTIC (round up with
+                            probability equal to the fraction of an FP16 ulp,
+                            from a hash of node id and step: unbiased, so
+                            increments below half an ulp are not lost)        */
 } plb_config;
+enum { PLB_ROUND_NEAREST = 0, PLB_ROUND_STOCHASTIC = 1 };
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
  * REFERENCE order/I32 index, 256 threads, one node per thread, device 0. */
@@ -161,10 +167,30 @@ int plb_read_macros(plb_ctx *c, double *rho, double *ux, double *uy, double *uz)
  * (shifted storage is un-shifted). */
 int plb_read_populations(plb_ctx *c, double *f_aos);
 
-/* Overwrites the populations (AoS [n][19], V order); used for
- * checkpoint/resume (the reference's save_state/recovery_state stubs,
- * porelb.c:367-373) and tests. */
+/* Overwrites the populations (AoS [n][19], V order) and clears the density
+ * guard; used by tests and to start from a given FP64 state.  With FP32/FP16
+ * storage the values are shifted and rounded to the stored type, so an
+ * FP64 round trip through plb_read_populations is exact for FP64 storage
+ * only: checkpoints use plb_read_state / plb_write_state. */
 int plb_write_populations(plb_ctx *c, const double *f_aos);
+
+/* Stored population state, bit for bit: checkpoint/resume (new; fills the
+ * empty save_state/recovery_state stubs of porelb.c:367-373).  The elements
+ * the step kernel keeps, AoS [n][19] in V order: FP64 populations (8 B),
+ * FP32 deviations f - w_j rho0 (4 B), or the FP16 bits of
+ * (f - w_j rho0) 2^k (2 B).  Writing back what plb_read_state gave, into a
+ * context of the same mask and precision, continues the run bit for bit.
+ * plb_write_state also clears the density guard. */
+typedef struct {
+    int precision;       /* PLB_FP64 | PLB_FP32 | PLB_FP16 */
+    int elem_bytes;      /* 8, 4 or 2 bytes per stored population */
+    int fp16_scale_log2; /* k of the FP16 storage scale 2^k (0 otherwise) */
+    double rho0;         /* the shift w_j rho0 of FP32/FP16 storage */
+    int64_t n_nodes;     /* the state holds n_nodes * 19 elements */
+} plb_state_info;
+int plb_get_state_info(const plb_ctx *c, plb_state_info *out);
+int plb_read_state(plb_ctx *c, void *buf);
+int plb_write_state(plb_ctx *c, const void *buf);
 
 /* The pull table [18][n] in V order with V-order node ids; entry codes:
  * >= 0 source node (pull direction j from it), -1 bounce (own node,
@@ -251,6 +277,16 @@ int plb_batch_create(plb_batch **out, const plb_config *cfg, const plb_sample *s
  * sample ran; updates[0] = total node updates performed; ms = device time. */
 int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
                   int64_t *steps_done, int64_t *updates, double *ms);
+/* The same run as one persistent launch: a resident grid (cooperative
+ * launch) advances every sample through the same steps, checks and
+ * convergence rule as plb_batch_run, with a grid barrier between steps and
+ * the per-sample u_x sums and stop decisions made on the device (no host
+ * round trip per check).  The per-sample sums use the same fixed order in
+ * both drivers, so both stop every sample at the same step and leave the
+ * same states.  A barrier wait longer than 2 s ends the launch and returns
+ * PLB_ERR_HIP. */
+int plb_batch_run_persistent(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
+                             int64_t *steps_done, int64_t *updates, double *ms);
 /* Per-sample reductions after a run (n_samples entries). */
 int plb_batch_reduce(plb_batch *b, plb_reduction *out);
 /* rho, u_x, u_y, u_z of one sample in its V order (z,y,x lexicographic, one

@@ -18,9 +18,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-# PLB_LIBRARY points at another build of the same library (e.g. a variant
-# built for timing experiments); default: the in-tree build.
-LIB_PATH = os.environ.get("PLB_LIBRARY") or os.path.join(HERE, "libplb.so")
+# Always the in-tree build: the library beside this file.
+LIB_PATH = os.path.join(HERE, "libplb.so")
 HEADER = os.path.join(REPO, "include", "plb.h")
 HEADERS = [HEADER, os.path.join(REPO, "include", "plb_tune.h")]
 BIN_DIR = os.path.join(HERE, "bin")
@@ -31,6 +30,7 @@ BB_REFCOMPAT, BB_HALFWAY = 0, 1
 LAYOUT_SOA, LAYOUT_AOSOA = 0, 1
 ORDER_REFERENCE, ORDER_BRICK, ORDER_PENCIL = 0, 1, 2
 INDEX_I32, INDEX_TILE8, INDEX_BITMAP = 0, 1, 2
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 TYPE_FLUID, TYPE_SOLID, TYPE_EXTERN = 0, 1, 2
 ERR_DENSITY = -6
 
@@ -52,7 +52,8 @@ class Config(C.Structure):
                 ("precision", C.c_int), ("bb_mode", C.c_int), ("layout", C.c_int),
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
                 ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int),
-                ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int), ("waves_per_simd", C.c_int)]
+                ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int), ("waves_per_simd", C.c_int),
+                ("fp16_rounding", C.c_int)]
 
 
 class Reduction(C.Structure):
@@ -67,6 +68,11 @@ class Fault(C.Structure):
     _fields_ = [("tripped", C.c_int), ("step", C.c_int64), ("node", C.c_int64), ("pid", C.c_int),
                 ("local_id", C.c_int64), ("x", C.c_int), ("y", C.c_int), ("z", C.c_int),
                 ("rho", C.c_double)]
+
+
+class StateInfo(C.Structure):
+    _fields_ = [("precision", C.c_int), ("elem_bytes", C.c_int), ("fp16_scale_log2", C.c_int),
+                ("rho0", C.c_double), ("n_nodes", C.c_int64)]
 
 
 class Sample(C.Structure):
@@ -99,6 +105,9 @@ def _lib():
             "plb_read_populations": ([p, p], i32),
             "plb_write_populations": ([p, p], i32),
             "plb_read_table": ([p, p], i32),
+            "plb_get_state_info": ([p, p], i32),
+            "plb_read_state": ([p, p], i32),
+            "plb_write_state": ([p, p], i32),
             "plb_num_nodes": ([p], i64),
             "plb_reduce": ([p, p], i32),
             "plb_last_fault": ([p, p], i32),
@@ -109,6 +118,7 @@ def _lib():
             "plb_preprocess": ([p, i32, i32, i32, i32, i32, p, p, p], i32),
             "plb_batch_create": ([p, p, p, i32], i32),
             "plb_batch_run": ([p, i64, i64, d, p, p, p], i32),
+            "plb_batch_run_persistent": ([p, i64, i64, d, p, p, p], i32),
             "plb_batch_reduce": ([p, p], i32),
             "plb_run_to_convergence": ([p, i64, i64, i64, d, p, p], i32),
             "plb_batch_read_macros": ([p, i32, p, p, p, p], i32),
@@ -294,6 +304,28 @@ class Lattice:
         assert f.shape == (self.N, Q)
         _check(_lib().plb_write_populations(self._h, _ptr(f)), "plb_write_populations")
 
+    STATE_DTYPE = {FP64: np.float64, FP32: np.float32, FP16: np.uint16}
+
+    def state_info(self):
+        si = StateInfo()
+        _check(_lib().plb_get_state_info(self._h, C.byref(si)), "plb_get_state_info")
+        return {k: getattr(si, k) for k, _ in si._fields_}
+
+    def read_state(self):
+        """The stored population elements (N, 19) in V order: float64
+        populations, float32 deviations, or the uint16 bits of FP16 scaled
+        deviations (plb_read_state)."""
+        info = self.state_info()
+        s = np.zeros((self.N, Q), dtype=self.STATE_DTYPE[info["precision"]])
+        _check(_lib().plb_read_state(self._h, _ptr(s)), "plb_read_state")
+        return s
+
+    def write_state(self, s):
+        info = self.state_info()
+        s = np.ascontiguousarray(s, dtype=self.STATE_DTYPE[info["precision"]])
+        assert s.shape == (self.N, Q)
+        _check(_lib().plb_write_state(self._h, _ptr(s)), "plb_write_state")
+
     def read_table(self):
         t = np.zeros((18, self.N), dtype=np.int32)
         _check(_lib().plb_read_table(self._h, _ptr(t)), "plb_read_table")
@@ -373,12 +405,15 @@ class Batch:
         self.n = len(self._flags)
         self.nodes = [int(_lib().plb_batch_num_nodes(h, s)) for s in range(self.n)]
 
-    def run(self, max_steps, check_every, tol):
+    def run(self, max_steps, check_every, tol, persistent=False):
+        """(steps per sample, node updates, device ms); persistent=True runs
+        plb_batch_run_persistent (one launch) instead of one launch per step."""
         steps = np.zeros(self.n, dtype=np.int64)
         upd = C.c_int64(0)
         ms = C.c_double(0.0)
-        _check(_lib().plb_batch_run(self._h, max_steps, check_every, tol, _ptr(steps), C.byref(upd),
-                                    C.byref(ms)), "plb_batch_run")
+        fn = _lib().plb_batch_run_persistent if persistent else _lib().plb_batch_run
+        _check(fn(self._h, max_steps, check_every, tol, _ptr(steps), C.byref(upd), C.byref(ms)),
+               "plb_batch_run_persistent" if persistent else "plb_batch_run")
         return steps, upd.value, ms.value
 
     def reduce(self):

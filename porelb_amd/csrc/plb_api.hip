@@ -43,6 +43,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->nodes_per_thread = 1;
     cfg->graph_steps = 0;
     cfg->waves_per_simd = 0;
+    cfg->fp16_rounding = PLB_ROUND_NEAREST;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -96,6 +97,8 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->nodes_per_thread == 2 && cfg->block_size > 512) return PLB_ERR_INVALID;
     if (cfg->graph_steps < 0 || cfg->graph_steps > 4096 || cfg->graph_steps % 2) return PLB_ERR_INVALID;
     if (cfg->waves_per_simd < 0 || cfg->waves_per_simd > 8) return PLB_ERR_INVALID;
+    if (cfg->fp16_rounding != PLB_ROUND_NEAREST && cfg->fp16_rounding != PLB_ROUND_STOCHASTIC)
+        return PLB_ERR_INVALID;
     return PLB_OK;
 }
 
@@ -118,6 +121,7 @@ int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, in
     c->elem = cfg->precision == PLB_FP64 ? 8 : (cfg->precision == PLB_FP32 ? 4 : 2);
     c->omega = 1.0 / cfg->tau;  // porelb.c:88
     const int k = cfg->fp16_scale_log2 ? cfg->fp16_scale_log2 : fp16_scale_log2_for(cfg->gx);
+    c->scale_log2 = cfg->precision == PLB_FP16 ? k : 0;
     c->scale = cfg->precision == PLB_FP16 ? ldexpf(1.f, k) : 1.f;
     c->inv_scale = cfg->precision == PLB_FP16 ? ldexpf(1.f, -k) : 1.f;
     int ndev = 0;
@@ -269,6 +273,16 @@ static unsigned long long read_fault_key(plb_ctx* c, int* rc)
         *rc = PLB_ERR_HIP;
     }
     return key;
+}
+
+// A state written in from outside starts with a clear density guard: the
+// fault word is reset and nothing is pending.
+static int clear_fault(plb_ctx* c)
+{
+    PLB_HIP(hipMemsetAsync(c->fault, 0xff, sizeof(unsigned long long), c->stream));
+    PLB_HIP(hipStreamSynchronize(c->stream));
+    c->pending_n = 0;
+    return PLB_OK;
 }
 
 extern "C" int plb_step_async(plb_ctx* c, int64_t it0, int64_t nsteps)
@@ -424,6 +438,57 @@ extern "C" int plb_write_populations(plb_ctx* c, const double* f_aos)
         if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PLB_ERR_HIP;
     }
     (void)hipFree(d);
+    if (!rc) rc = clear_fault(c);
+    return rc;
+}
+
+extern "C" int plb_get_state_info(const plb_ctx* c, plb_state_info* out)
+{
+    if (!c || !out) return PLB_ERR_INVALID;
+    memset(out, 0, sizeof *out);
+    out->precision = c->cfg.precision;
+    out->elem_bytes = (int)c->elem;
+    out->fp16_scale_log2 = c->scale_log2;
+    out->rho0 = c->cfg.rho0;
+    out->n_nodes = c->N;
+    return PLB_OK;
+}
+
+// Stored elements between host AoS [n][19] and the device layout, through a
+// device staging chunk.
+static int copy_state(plb_ctx* c, void* host, bool to_device)
+{
+    if (!c || (!host && c->N)) return PLB_ERR_INVALID;
+    PLB_HIP(hipSetDevice(c->device));
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(c->N, 1), CHUNK_NODES);
+    const size_t rec = Q * c->elem;
+    void* d = nullptr;
+    PLB_HIP(hipMalloc(&d, rec * chunk));
+    int rc = PLB_OK;
+    char* h = (char*)host;
+    for (int64_t b = 0; b < c->N && !rc; b += chunk) {
+        const int64_t n = std::min(chunk, c->N - b);
+        if (to_device) {
+            if (hipMemcpyAsync(d, h + b * rec, rec * n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+                rc = PLB_ERR_HIP;
+            if (!rc) rc = launch_copy_state(c, b, n, d, true);
+        } else {
+            rc = launch_copy_state(c, b, n, d, false);
+            if (!rc && hipMemcpyAsync(h + b * rec, d, rec * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                rc = PLB_ERR_HIP;
+        }
+        if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = PLB_ERR_HIP;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+extern "C" int plb_read_state(plb_ctx* c, void* buf) { return copy_state(c, buf, false); }
+
+extern "C" int plb_write_state(plb_ctx* c, const void* buf)
+{
+    int rc = copy_state(c, const_cast<void*>(buf), true);
+    if (!rc) rc = clear_fault(c);
     return rc;
 }
 
@@ -512,6 +577,7 @@ extern "C" int plb_last_fault(plb_ctx* c, plb_fault* out)
     out->tripped = 1;
     out->step = (int64_t)(key >> 32);
     out->node = (int64_t)(key & 0xffffffffull);
+    if (out->node >= c->N) return PLB_ERR_BOUNDS;
     int p = 0;
     for (int q = 0; q < c->num_p; q++)
         if (c->V_sub_start[q] <= out->node) p = q;
@@ -521,6 +587,7 @@ extern "C" int plb_last_fault(plb_ctx* c, plb_fault* out)
     if (c->inv_perm) {
         uint32_t kk = 0;
         PLB_HIP(hipMemcpy(&kk, c->inv_perm + out->node, sizeof kk, hipMemcpyDeviceToHost));
+        if ((int64_t)kk >= c->N) return PLB_ERR_BOUNDS;
         k = kk;
     }
     uint64_t xyz = 0;

@@ -12,6 +12,7 @@
 // check step, so after a run they hold each sample's last step; the batch
 // context's inv_perm maps each sample's V order into the concatenation.
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -38,6 +39,7 @@ struct plb_batch {
     double* d_omega = nullptr;
     double* d_gx = nullptr;
     double* d_sums = nullptr;
+    double* d_partial = nullptr;  // per-chunk u_x partial sums
 };
 
 extern "C" void plb_batch_destroy(plb_batch* b)
@@ -48,7 +50,7 @@ extern "C" void plb_batch_destroy(plb_batch* b)
         (void)hipStreamSynchronize(b->c->stream);
     }
     void* ptrs[] = {b->d_block_sample, b->d_active, b->d_end, b->d_off, b->d_n,
-                    b->d_omega, b->d_gx, b->d_sums};
+                    b->d_omega, b->d_gx, b->d_sums, b->d_partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     plb_destroy(b->c);
@@ -191,6 +193,8 @@ extern "C" int plb_batch_create(plb_batch** out, const plb_config* cfg, const pl
         if (e == hipSuccess) e = hipMalloc(&b->d_omega, sizeof(double) * n_samples);
         if (e == hipSuccess) e = hipMalloc(&b->d_gx, sizeof(double) * n_samples);
         if (e == hipSuccess) e = hipMalloc(&b->d_sums, sizeof(double) * n_samples);
+        if (e == hipSuccess) e = hipMalloc(&b->d_partial, sizeof(double) * nblocks);
+        if (e == hipSuccess) e = hipMemset(b->d_partial, 0, sizeof(double) * nblocks);
         if (e == hipSuccess)
             e = hipMemcpy(b->d_block_sample, b->block_sample.data(), sizeof(int32_t) * nblocks,
                           hipMemcpyHostToDevice);
@@ -277,7 +281,8 @@ extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_ever
                 done[s] += m;
                 upd += m * b->n[s];
             }
-        rc = launch_sample_sums(c, b->d_off, b->d_n, b->S, b->d_sums);
+        rc = launch_sample_sums(c, b->d_active, n_active, b->d_block_sample, b->d_end, b->d_off, b->d_n, b->S,
+                                b->d_partial, b->d_sums);
         if (rc) return rc;
         unsigned long long key = NO_FAULT;
         PLB_HIP(hipMemcpyAsync(sums.data(), b->d_sums, sizeof(double) * b->S, hipMemcpyDeviceToHost,
@@ -313,6 +318,109 @@ extern "C" int plb_batch_run(plb_batch* b, int64_t max_steps, int64_t check_ever
     if (ms) *ms = t;
     if (steps_done)
         for (int s = 0; s < b->S; s++) steps_done[s] = done[s];
+    if (updates) *updates = upd;
+    return PLB_OK;
+}
+
+// The whole run as one persistent launch (k_batch_persistent): the same
+// steps, check schedule, convergence rule and per-sample sums as
+// plb_batch_run, with the checks made on the device.
+extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t check_every, double tol,
+                                        int64_t* steps_done, int64_t* updates, double* ms)
+{
+    if (!b || max_steps < 0 || check_every < 1 || !(tol >= 0.0)) return PLB_ERR_INVALID;
+    Range range("plb_batch_run_persistent");
+    plb_ctx* c = b->c;
+    PLB_HIP(hipSetDevice(c->device));
+    int rc = plb_init_equilibrium(c);
+    if (rc) return rc;
+    const int S = b->S;
+    const int bs = c->cfg.block_size;
+    std::vector<int> act(S), cur(S, c->cur);
+    std::vector<int64_t> steps(S, 0);
+    std::vector<double> prev(S, NAN);
+    int n_active = 0;
+    for (int s = 0; s < S; s++) {
+        act[s] = b->n[s] > 0 ? 1 : 0;
+        n_active += act[s];
+    }
+    // per-run device state: active, cur, steps, prev, n_active, abort, barrier counter
+    DevBuf<int> d_act, d_cur, d_flags;
+    DevBuf<int64_t> d_steps;
+    DevBuf<double> d_prev;
+    DevBuf<unsigned long long> d_bar;
+    PLB_HIP(d_act.alloc(S));
+    PLB_HIP(d_cur.alloc(S));
+    PLB_HIP(d_steps.alloc(S));
+    PLB_HIP(d_prev.alloc(S));
+    PLB_HIP(d_flags.alloc(2));
+    PLB_HIP(d_bar.alloc(1));
+    const int flags[2] = {n_active, 0};
+    PLB_HIP(hipMemcpyAsync(d_act, act.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
+    PLB_HIP(hipMemcpyAsync(d_cur, cur.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
+    PLB_HIP(hipMemcpyAsync(d_steps, steps.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice, c->stream));
+    PLB_HIP(hipMemcpyAsync(d_prev, prev.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+    PLB_HIP(hipMemcpyAsync(d_flags, flags, sizeof flags, hipMemcpyHostToDevice, c->stream));
+    PLB_HIP(hipMemsetAsync(d_bar, 0, sizeof(unsigned long long), c->stream));
+    PersistArgs p;
+    p.f[0] = c->f[0];
+    p.f[1] = c->f[1];
+    p.cur0 = c->cur;
+    p.nchunks = c->Npad / bs;
+    p.max_steps = max_steps;
+    p.check_every = check_every;
+    p.tol = tol;
+    p.nsamples = S;
+    p.chunk_sample = b->d_block_sample;
+    p.sample_end = b->d_end;
+    p.sample_off = b->d_off;
+    p.sample_n = b->d_n;
+    p.omega = b->d_omega;
+    p.gx = b->d_gx;
+    p.active = d_act;
+    p.cur = d_cur;
+    p.steps = d_steps;
+    p.prev = d_prev;
+    p.partial = b->d_partial;
+    p.bar = d_bar;
+    p.n_active = d_flags.p;
+    p.abort = d_flags.p + 1;
+    int clock_khz = 0;
+    PLB_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device));
+    p.wait_ticks = (uint64_t)(clock_khz > 0 ? clock_khz : 100000) * 2000ull;  // 2 s of wall_clock64
+    int blocks = 0;
+    PLB_HIP(hipEventRecord(c->ev0, c->stream));
+    if (n_active > 0 && max_steps > 0) {
+        rc = launch_batch_persistent(c, p, &blocks);
+        if (rc) return rc;
+    }
+    PLB_HIP(hipEventRecord(c->ev1, c->stream));
+    PLB_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    PLB_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    int hflags[2] = {0, 0};
+    unsigned long long key = NO_FAULT;
+    PLB_HIP(hipMemcpy(hflags, d_flags, sizeof hflags, hipMemcpyDeviceToHost));
+    PLB_HIP(hipMemcpy(&key, c->fault, sizeof key, hipMemcpyDeviceToHost));
+    PLB_HIP(hipMemcpy(cur.data(), d_cur, sizeof(int) * S, hipMemcpyDeviceToHost));
+    PLB_HIP(hipMemcpy(steps.data(), d_steps, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
+    if (hflags[1]) {
+        fprintf(stderr, "libplb: persistent batch run: a grid barrier wait exceeded 2 s (%d blocks)\n", blocks);
+        return PLB_ERR_HIP;
+    }
+    if (key != NO_FAULT) return PLB_ERR_DENSITY;
+    int64_t upd = 0;
+    for (int s = 0; s < S; s++) {
+        b->sample_cur[s] = cur[s];
+        upd += steps[s] * b->n[s];
+    }
+    // the context's buffer parity after the last step any sample ran
+    int64_t smax = 0;
+    for (int s = 0; s < S; s++) smax = std::max(smax, steps[s]);
+    c->cur = (int)((c->cur + smax) & 1);
+    if (ms) *ms = t;
+    if (steps_done)
+        for (int s = 0; s < S; s++) steps_done[s] = steps[s];
     if (updates) *updates = upd;
     return PLB_OK;
 }

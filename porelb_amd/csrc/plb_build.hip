@@ -413,6 +413,14 @@ int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, cons
     PLB_HIP(hipGetLastError());
     int rc = exclusive_scan_u64(cnt, wpre, nw, s);
     if (rc) return rc;
+    // Every node must own a voxel of its own: two records with the same
+    // coordinates would get the same internal id, race on its table row and
+    // leave another id unassigned.  The set bits count the distinct voxels.
+    uint64_t last[2] = {0, 0};
+    PLB_HIP(hipMemcpyAsync(&last[0], wpre.p + nw - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipMemcpyAsync(&last[1], cnt.p + nw - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PLB_HIP(hipStreamSynchronize(s));
+    if ((int64_t)(last[0] + last[1]) != N) return PLB_ERR_BOUNDS;
     PLB_HIP(inv.alloc(N));
     PLB_HIP(p.alloc(N));
     if (!xyz_int) PLB_HIP(xyz_tmp.alloc(N));

@@ -134,7 +134,8 @@ struct plb_ctx {
     double* red = nullptr;      // reduction partials
     int red_blocks = 0;
     double omega = 0.0;
-    float scale = 1.f, inv_scale = 1.f;  // FP16 storage scale
+    float scale = 1.f, inv_scale = 1.f;  // FP16 storage scale 2^scale_log2
+    int scale_log2 = 0;
     int64_t last_it = -1;       // last step issued
     int64_t pending_it0 = 0, pending_n = 0;
     int cur_before = 0;
@@ -187,6 +188,31 @@ struct DevBuf {
 
 // Internal entry points shared between translation units.
 namespace plb {
+// Arguments of the persistent batch kernel (plb_step.hip, k_batch_persistent).
+struct PersistArgs {
+    void* f[2];
+    int cur0;                     // buffer holding every sample's state at step 0
+    int64_t nchunks;
+    int64_t max_steps, check_every;
+    double tol;
+    int nsamples;
+    const int32_t* chunk_sample;  // chunk -> sample
+    const int64_t* sample_end;    // first node after each sample
+    const int64_t* sample_off;
+    const int64_t* sample_n;
+    const double* omega;
+    const double* gx;
+    int* active;                  // per sample: 1 while running
+    int* cur;                     // per sample: buffer of its state after its last step
+    int64_t* steps;               // per sample: steps run
+    double* prev;                 // per sample: <u_x> at the previous check (NaN before)
+    double* partial;              // per chunk: u_x sum on check steps
+    unsigned long long* bar;      // arrivals at grid barriers (monotonic)
+    int* abort;                   // a barrier wait timed out
+    int* n_active;                // samples still running
+    uint64_t wait_ticks;          // wall_clock64 ticks a barrier wait may take
+};
+int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int* blocks);
 int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n);
 int build_table_from_flag(plb_ctx* c, const uint8_t* flag);
 int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_base = nullptr);
@@ -206,11 +232,13 @@ int launch_init_eq(plb_ctx* c);
 int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout);
 int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din);
 int launch_read_macros(plb_ctx* c, int comp, int64_t first, int64_t n, double* dout);
+int launch_copy_state(plb_ctx* c, int64_t first, int64_t n, void* aos, bool to_device);
 int launch_read_table(plb_ctx* c, int64_t first, int64_t n, int32_t* dout);
 int launch_reduce_range(plb_ctx* c, int buf, int64_t first, int64_t n, bool with_mass,
                         double* out2_host);
-int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,
-                       double* d_out);
+int launch_sample_sums(plb_ctx* c, const int32_t* d_active, int n_active, const int32_t* chunk_sample,
+                       const int64_t* sample_end, const int64_t* d_off, const int64_t* d_n, int nsamples,
+                       double* d_partial, double* d_out);
 int check_cfg(const plb_config* cfg);
 int fp16_scale_log2_for(double gx);
 int compress_tile8(plb_ctx* c);

@@ -50,6 +50,7 @@ struct StepArgs {
     FastDiv div_nx, div_py, div_nz;
     // Graph replays: the step number is it + *it_base (set before each replay).
     const int64_t* it_base;
+    int stochastic;  // FP16 stores: stochastic rounding
 };
 
 __device__ __forceinline__ StepArgs with_step(const StepArgs& a)
@@ -289,6 +290,30 @@ __device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
     __builtin_nontemporal_store(b, (unsigned short*)p + k);
 }
 
+// Stochastic rounding of FP16 stores.  v_cvt_sr_f16_f32 rounds up with
+// probability equal to the dropped fraction of an FP16 ulp, taken from the
+// random word r; the expected stored value equals the FP32 value, so the
+// sub-ulp increments of a flow near steady state accumulate instead of
+// being rounded away.  r for direction j of a node: a 32-bit hash of the
+// node id and the step (lowbias32, C. Wellons, "Prospecting for hash
+// functions", 2018) plus j times the golden-ratio constant, which keeps every
+// direction's word uniformly distributed.
+typedef _Float16 plb_half2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, float sc, uint32_t r)
+{
+    const plb_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(plb_half2{0, 0}, v * sc, r, false);
+    __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, h[0]), (unsigned short*)p + k);
+}
+
 template <typename S, int LAYOUT>
 __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, const int32_t t[Q],
                                               const S raw[Q], unsigned long long fk)
@@ -334,6 +359,9 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     const float fu = -c2 * u;  // force term c2 (e_jx - u) for e_jx = 0, then +-c2
     const float sc = a.scale;
     S* __restrict__ onode = fout + fpos<LAYOUT>(0, i, a.Npad);
+    uint32_t rnd = 0;
+    if constexpr (sizeof(S) == 2)
+        if (a.stochastic) rnd = mix32((uint32_t)i ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
 #pragma unroll
     for (int j = 0; j < Q; j++) {
         float eu = 0.f;
@@ -346,7 +374,14 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
         const float feq = fmaf(wj, rho0, dfeq);
         const float frc = EX[j] > 0 ? fu + c2 : (EX[j] < 0 ? fu - c2 : fu);
         const float o = fmaf(c1, d[j], fmaf(omega, dfeq, frc * feq));
-        st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
+        if constexpr (sizeof(S) == 2) {
+            if (a.stochastic)
+                st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc, rnd + (uint32_t)j * 0x9e3779b9u);
+            else
+                st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
+        } else {
+            st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
+        }
     }
 }
 
@@ -431,6 +466,140 @@ __global__ void __launch_bounds__(1024) k_step_batch(StepArgs a, const BatchArgs
     node_any<S, LAYOUT, IDX>(a, i, *a.fault);
 }
 
+// ---------------------------------------------------------------- fixed-order sums
+// Sum of one double per thread of the block, in a fixed order; the result
+// is valid in thread 0.  lds holds blockDim.x / 64 doubles.
+__device__ __forceinline__ double block_sum_fixed(double v, double* lds)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += lds[w];
+    __syncthreads();
+    return t;
+}
+
+// Sum of a sample's chunk partials by one wave (lane 0 holds the result).
+__device__ __forceinline__ double wave_sum_chunks(const double* partial, int64_t c0, int64_t nc)
+{
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int64_t k = lane; k < nc; k += 64) acc += partial[c0 + k];
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    return acc;
+}
+
+// ---------------------------------------------------------------- persistent batch
+// One launch runs every sample of a batch from its first step to its own
+// convergence stop.  The grid is resident (cooperative launch); each
+// work-group walks the chunks (blocks of the batch layout) of the active
+// samples, with each XCD taking contiguous runs of chunks as in xcd_block.
+// A grid barrier separates time steps.  On check steps every chunk stores
+// its macros and the fixed-order sum of its u_x; after a barrier, one wave
+// per sample forms the sample's sum (the same order as k_sample_sums),
+// applies the convergence rule and retires the sample; a second barrier
+// publishes the decisions.
+//
+// Barriers use agent-scope release/acquire fences around a monotonic 64-bit
+// arrival counter.  A wait is bounded by wall time (wall_clock64 ticks):
+// a work-group that waits longer sets `abort`, every work-group leaves the
+// loop at its next barrier, and the host reports the failure, so the kernel
+// always ends.
+
+__device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long long& target)
+{
+    __shared__ int ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    target += gridDim.x;
+    if (threadIdx.x == 0) {
+        int good = 1;
+        __hip_atomic_fetch_add(p.bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(p.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                good = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > p.wait_ticks) {
+                __hip_atomic_store(p.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        ok = good;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
+}
+
+template <typename S, int LAYOUT, int IDX>
+__global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const PersistArgs p)
+{
+    __shared__ double lds[16];
+    const int64_t G = gridDim.x;
+    const int64_t me = xcd_block(blockIdx.x, G);  // each XCD: a contiguous run of chunks per round
+    unsigned long long target = 0;
+    const int bs = blockDim.x;
+    for (int64_t t = 0; t < p.max_steps; t++) {
+        const bool check = (t + 1) % p.check_every == 0 || t + 1 == p.max_steps;
+        const int par = (p.cur0 + (int)(t & 1)) & 1;
+        a.fin = p.f[par];
+        a.fout = p.f[par ^ 1];
+        a.it = t;
+        a.write_macros = check ? 1 : 0;
+        for (int64_t chunk = me; chunk < p.nchunks; chunk += G) {
+            const int s = p.chunk_sample[chunk];
+            // flags other work-groups wrote before the last barrier: coherent loads
+            if (!__hip_atomic_load(p.active + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+            const int64_t i = chunk * bs + threadIdx.x;
+            const bool live = i < p.sample_end[s];
+            if (live) {
+                a.omega = p.omega[s];
+                a.gx = p.gx[s];
+                node_any<S, LAYOUT, IDX>(a, i, NO_FAULT);
+            }
+            if (check) {
+                const double t_sum = block_sum_fixed(live ? a.macros[a.Npad + i] : 0.0, lds);
+                if (threadIdx.x == 0) p.partial[chunk] = t_sum;
+            }
+        }
+        if (!grid_barrier(p, target)) return;
+        // Density guard: stop after the faulting step.  Only a fault of this
+        // step or earlier counts: a faster work-group may already be in the
+        // next step, and every work-group must leave at the same barrier.
+        const unsigned long long fk = __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (fk != NO_FAULT && (int64_t)(fk >> 32) <= t) return;
+        if (!check) continue;
+        if (threadIdx.x < 64) {
+            for (int64_t s = me; s < p.nsamples; s += G) {
+                if (!__hip_atomic_load(p.active + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+                const double sum = wave_sum_chunks(p.partial, p.sample_off[s] / bs, (p.sample_n[s] + bs - 1) / bs);
+                if (threadIdx.x == 0) {
+                    const double mean = sum / (double)p.sample_n[s];
+                    const double scale = fmax(fabs(mean), 1e-3 * fabs(p.gx[s]));
+                    const double prev = p.prev[s];
+                    p.steps[s] = t + 1;
+                    p.cur[s] = par ^ 1;
+                    if (!isnan(prev) && fabs(mean - prev) <= p.tol * scale) {
+                        __hip_atomic_store(p.active + s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(p.n_active, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    p.prev[s] = mean;
+                }
+            }
+        }
+        if (!grid_barrier(p, target)) return;
+        if (__hip_atomic_load(p.n_active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    }
+}
+
 // ---------------------------------------------------------------- init / IO kernels
 template <typename S, int LAYOUT>
 __global__ void k_init_eq(void* f, int64_t N, int64_t Npad, double rho0)
@@ -491,6 +660,28 @@ __global__ void k_write_pop(void* f, int64_t Npad, const uint32_t* inv_perm, int
             ((float*)f)[p] = (float)(v - TP[j] * rho0);
         else
             ((__half*)f)[p] = __float2half_rn((float)(v - TP[j] * rho0) * sc);
+    }
+}
+
+// Stored elements as they are, AoS [n][19] in V order <-> device layout
+// (checkpoints: no conversion, so a written-back state is bit-identical).
+template <typename S, int LAYOUT>
+__global__ void k_copy_state(void* f, int64_t Npad, const uint32_t* inv_perm, int64_t first, int64_t n,
+                             void* aos_bytes, int to_device)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    S* dev = (S*)f;
+    S* aos = (S*)aos_bytes;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const int64_t p = fpos<LAYOUT>(j, k, Npad);
+        if (to_device)
+            dev[p] = aos[r * Q + j];
+        else
+            aos[r * Q + j] = dev[p];
     }
 }
 
@@ -596,21 +787,34 @@ k_reduce_partial(const void* f, const double* ux, int64_t first, int64_t n, int6
     }
 }
 
-// Per-sample sum of ux over each sample's node range (one block per sample).
-__global__ void __launch_bounds__(RED_T)
-k_sample_sums(const double* ux, const int64_t* off, const int64_t* n, double* out)
+// ---------------------------------------------------------------- per-sample u_x sums (batches)
+// A sample's sum of u_x is formed in one fixed order by both batch drivers
+// (one launch per step, and the persistent launch): per chunk (one block of
+// the batch layout) a fixed tree over its nodes, then per sample the chunk
+// partials, lane l of one wave taking chunks l, l+64, ... in turn, and a
+// fixed butterfly over the 64 lanes.  Both drivers therefore see the same
+// sums bit for bit and stop every sample at the same step.
+
+// partial[chunk] = sum of ux over the chunk's nodes, for the listed chunks.
+__global__ void __launch_bounds__(1024)
+k_chunk_partials(const double* ux, const int32_t* chunks, const int32_t* chunk_sample,
+                 const int64_t* sample_end, double* partial)
 {
-    __shared__ double sm[RED_T];
+    __shared__ double lds[16];
+    const int64_t chunk = chunks[blockIdx.x];
+    const int64_t i = chunk * blockDim.x + threadIdx.x;
+    const double v = i < sample_end[chunk_sample[chunk]] ? ux[i] : 0.0;
+    const double t = block_sum_fixed(v, lds);
+    if (threadIdx.x == 0) partial[chunk] = t;
+}
+
+// out[s] = sum of sample s's chunk partials (one wave per sample).
+__global__ void __launch_bounds__(64)
+k_sample_sums(const double* partial, const int64_t* off, const int64_t* n, int bs, double* out)
+{
     const int s = blockIdx.x;
-    double acc = 0.0;
-    for (int64_t r = threadIdx.x; r < n[s]; r += RED_T) acc += ux[off[s] + r];
-    sm[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = RED_T / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[s] = sm[0];
+    const double t = wave_sum_chunks(partial, off[s] / bs, (n[s] + bs - 1) / bs);
+    if (threadIdx.x == 0) out[s] = t;
 }
 
 __global__ void __launch_bounds__(RED_T) k_reduce_final(double* partial, int nb)
@@ -715,6 +919,7 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.div_py = make_fastdiv(a.py);
     a.div_nz = make_fastdiv(a.nz);
     a.it_base = nullptr;
+    a.stochastic = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC;
     return a;
 }
 
@@ -813,6 +1018,47 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
 }
 
 
+// The persistent batch run as one cooperative launch of as many blocks as can
+// be resident at once (at most one per chunk).  *blocks = the grid size.
+template <typename S, int LAYOUT, int IDX>
+static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int* blocks)
+{
+    auto kern = k_batch_persistent<S, LAYOUT, IDX>;
+    const int bs = c->cfg.block_size;
+    const unsigned lds = step_lds_bytes(c, bs);
+    int per_cu = 0, cus = 0;
+    PLB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, bs, lds));
+    PLB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int64_t g = std::min<int64_t>((int64_t)per_cu * cus, p.nchunks);
+    if (g < 1) return PLB_ERR_INVALID;
+    StepArgs a = make_args(c, 0, false);
+    PersistArgs pp = p;
+    void* args[] = {(void*)&a, (void*)&pp};
+    PLB_HIP(hipLaunchCooperativeKernel((const void*)kern, dim3((unsigned)g), dim3(bs), args, lds, c->stream));
+    *blocks = (int)g;
+    return PLB_OK;
+}
+
+int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int* blocks)
+{
+    const int pr = c->cfg.precision, lay = c->cfg.layout, idx = c->cfg.index_kind;
+#define PLB_PERSIST(ST, L, I) if (pr == (sizeof(ST) == 8 ? PLB_FP64 : sizeof(ST) == 4 ? PLB_FP32 : PLB_FP16) && lay == L && idx == I) return launch_persistent_t<ST, L, I>(c, p, blocks)
+    PLB_PERSIST(double, PLB_LAYOUT_SOA, PLB_INDEX_I32);
+    PLB_PERSIST(double, PLB_LAYOUT_SOA, PLB_INDEX_TILE8);
+    PLB_PERSIST(double, PLB_LAYOUT_AOSOA, PLB_INDEX_I32);
+    PLB_PERSIST(double, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8);
+    PLB_PERSIST(float, PLB_LAYOUT_SOA, PLB_INDEX_I32);
+    PLB_PERSIST(float, PLB_LAYOUT_SOA, PLB_INDEX_TILE8);
+    PLB_PERSIST(float, PLB_LAYOUT_AOSOA, PLB_INDEX_I32);
+    PLB_PERSIST(float, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8);
+    PLB_PERSIST(__half, PLB_LAYOUT_SOA, PLB_INDEX_I32);
+    PLB_PERSIST(__half, PLB_LAYOUT_SOA, PLB_INDEX_TILE8);
+    PLB_PERSIST(__half, PLB_LAYOUT_AOSOA, PLB_INDEX_I32);
+    PLB_PERSIST(__half, PLB_LAYOUT_AOSOA, PLB_INDEX_TILE8);
+#undef PLB_PERSIST
+    return PLB_ERR_INVALID;
+}
+
 int launch_init_eq(plb_ctx* c)
 {
     for (int b = 0; b < 2; b++)
@@ -836,6 +1082,14 @@ int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din)
     PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_write_pop, dim3(nblk(n, 256)), dim3(256), 0,
                  c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, c->cfg.rho0, c->scale,
                  din);
+    PLB_HIP(hipGetLastError());
+    return PLB_OK;
+}
+
+int launch_copy_state(plb_ctx* c, int64_t first, int64_t n, void* aos, bool to_device)
+{
+    PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_copy_state, dim3(nblk(n, 256)), dim3(256), 0,
+                 c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, aos, to_device ? 1 : 0);
     PLB_HIP(hipGetLastError());
     return PLB_OK;
 }
@@ -893,11 +1147,15 @@ int launch_reduce(plb_ctx* c, double* out2)
     return launch_reduce_range(c, c->cur, 0, c->N, true, out2);
 }
 
-int launch_sample_sums(plb_ctx* c, const int64_t* d_off, const int64_t* d_n, int nsamples,
-                       double* d_out)
+int launch_sample_sums(plb_ctx* c, const int32_t* d_active, int n_active, const int32_t* chunk_sample,
+                       const int64_t* sample_end, const int64_t* d_off, const int64_t* d_n, int nsamples,
+                       double* d_partial, double* d_out)
 {
-    hipLaunchKernelGGL(k_sample_sums, dim3(nsamples), dim3(RED_T), 0, c->stream, c->macros + c->Npad,
-                       d_off, d_n, d_out);
+    if (n_active > 0)
+        hipLaunchKernelGGL(k_chunk_partials, dim3(n_active), dim3(c->cfg.block_size), 0, c->stream,
+                           c->macros + c->Npad, d_active, chunk_sample, sample_end, d_partial);
+    hipLaunchKernelGGL(k_sample_sums, dim3(nsamples), dim3(64), 0, c->stream, d_partial, d_off, d_n,
+                       c->cfg.block_size, d_out);
     PLB_HIP(hipGetLastError());
     return PLB_OK;
 }

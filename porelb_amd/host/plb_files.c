@@ -22,7 +22,7 @@ int plb_write_flag(const char *path, int64_t nvox, const uint8_t *flag)
     return put == (size_t)nvox ? 0 : -1;
 }
 
-int plb_read_vfile(const char *path, plb_vfile *v)
+int plb_read_vfile_header(const char *path, plb_vfile *v)
 {
     v->V_sub_start = NULL;
     v->V = NULL;
@@ -43,20 +43,47 @@ int plb_read_vfile(const char *path, plb_vfile *v)
         return -5;
     }
     v->V_sub_start = (int32_t *)calloc((size_t)v->num_p, sizeof(int32_t));
-    v->V = (plb_node_info *)malloc(sizeof(plb_node_info) * (size_t)(v->num_node ? v->num_node : 1));
-    if (!v->V_sub_start || !v->V) {
+    if (!v->V_sub_start) {
         fclose(fp);
-        plb_free_vfile(v);
         return -4;
     }
-    if (fread(v->V_sub_start, sizeof(int32_t), (size_t)v->num_p, fp) != (size_t)v->num_p ||
-        fread(v->V, sizeof(plb_node_info), (size_t)v->num_node, fp) != (size_t)v->num_node) {
+    if (fread(v->V_sub_start, sizeof(int32_t), (size_t)v->num_p, fp) != (size_t)v->num_p) {
         fclose(fp);
         plb_free_vfile(v);
         return -5;
     }
     fclose(fp);
     return 0;
+}
+
+int plb_read_vfile_records(const char *path, plb_vfile *v)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return -1;
+    v->V = (plb_node_info *)malloc(sizeof(plb_node_info) * (size_t)(v->num_node ? v->num_node : 1));
+    if (!v->V) {
+        fclose(fp);
+        return -4;
+    }
+    /* the records follow the 5-word header and V_sub_start */
+    if (fseek(fp, (long)(sizeof(int32_t) * (5 + (size_t)v->num_p)), SEEK_SET) != 0 ||
+        fread(v->V, sizeof(plb_node_info), (size_t)v->num_node, fp) != (size_t)v->num_node) {
+        fclose(fp);
+        free(v->V);
+        v->V = NULL;
+        return -5;
+    }
+    fclose(fp);
+    return 0;
+}
+
+int plb_read_vfile(const char *path, plb_vfile *v)
+{
+    int rc = plb_read_vfile_header(path, v);
+    if (rc) return rc;
+    rc = plb_read_vfile_records(path, v);
+    if (rc) plb_free_vfile(v);
+    return rc;
 }
 
 int plb_write_vfile(const char *path, const plb_vfile *v)

@@ -22,6 +22,10 @@ int plb_read_flag(const char *path, int64_t nvox, uint8_t *out);
 int plb_write_flag(const char *path, int64_t nvox, const uint8_t *flag);
 /* reads every rank's records; 0, -1 (open), -5 (short read), -4 (memory) */
 int plb_read_vfile(const char *path, plb_vfile *v);
+/* the same in two parts: the header and V_sub_start only (v->V = NULL), then
+ * the records of a vfile whose header is already read */
+int plb_read_vfile_header(const char *path, plb_vfile *v);
+int plb_read_vfile_records(const char *path, plb_vfile *v);
 int plb_write_vfile(const char *path, const plb_vfile *v);
 void plb_free_vfile(plb_vfile *v);
 /* part_%03d_%07d.bin for rank pid; 0 or -3 (open) */

@@ -30,9 +30,12 @@
  *   rule of plb_run_to_convergence); the final part files carry the step reached
  *   PLB_ASYNC_SAVE 1|0 (1): part files are written by a worker thread from one
  *   of two host buffers while the GPU keeps stepping (same files)
- *   PLB_CHECKPOINT file: populations written after the last step
- *   PLB_RESTART file: start from such a checkpoint (fills the empty
- *   save_state/recovery_state stubs of porelb.c:367-373)
+ *   PLB_CHECKPOINT file: the stored population state written after the last
+ *   step, bit for bit in every precision (header: step, node count,
+ *   precision, FP16 scale, rho0)
+ *   PLB_RESTART file: start from such a checkpoint; it must come from the
+ *   same V.bin, precision and rho0 (else exit -5).  Together they fill the
+ *   empty save_state/recovery_state stubs of porelb.c:367-373
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -73,7 +76,18 @@ static double wtime(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-#define CKPT_MAGIC 0x504c4243u /* "PLBC" */
+/* Checkpoint file: this header, then the stored elements (plb_read_state),
+ * AoS [N][19] in V order, elem_bytes each. */
+#define CKPT_MAGIC 0x32424c50u /* "PLB2" */
+typedef struct {
+    uint32_t magic;
+    int32_t precision;       /* PLB_FP64 | PLB_FP32 | PLB_FP16 */
+    int32_t elem_bytes;      /* 8, 4 or 2 */
+    int32_t fp16_scale_log2; /* FP16 storage scale 2^k, else 0 */
+    int64_t step;            /* the next step to run */
+    int64_t n_nodes;
+    double rho0;             /* shift of FP32/FP16 storage */
+} ckpt_header;
 
 /* save_data (porelb.c:375-391) for every rank: the macros of step `it` go
  * to one host buffer, and its part files are written either at once or by a
@@ -145,8 +159,10 @@ int main(int argc, char *argv[])
     }
     int it_max = atoi(argv[1]);
 
+    /* The header first (porelb.c:24-46): the rank-count check and the exit of
+     * ranks other than 0 come before the records are read. */
     plb_vfile v;
-    int rc = plb_read_vfile("V.bin", &v);
+    int rc = plb_read_vfile_header("V.bin", &v);
     if (rc == -1) {
         fprintf(stderr, "Node map file openning error!\n");
         exit(-1);
@@ -162,6 +178,11 @@ int main(int argc, char *argv[])
         exit(-2);
     }
     if (rank != 0) return 0;
+    rc = plb_read_vfile_records("V.bin", &v);
+    if (rc) {
+        fprintf(stderr, "%d >>> Error read V \n", rank);
+        exit(rc);
+    }
     for (int p = 0; p < v.num_p; p++) {
         int64_t e = p + 1 < v.num_p ? v.V_sub_start[p + 1] : v.num_node;
         fprintf(stderr, "%d >>> My num of fluid nodes: %d \n", p, (int)(e - v.V_sub_start[p]));
@@ -260,19 +281,32 @@ int main(int argc, char *argv[])
     int it0 = 0;
     const char *restart = getenv("PLB_RESTART");
     if (restart && *restart) {
+        plb_state_info si;
+        plb_get_state_info(c, &si);
+        ckpt_header h;
+        void *f = plb_host_alloc((size_t)si.elem_bytes * 19 * (size_t)(N ? N : 1));
         FILE *fp = fopen(restart, "rb");
-        uint32_t magic = 0;
-        int64_t hdr[2] = {0, 0};
-        double *f = (double *)plb_host_alloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
-        if (!f || !fp || fread(&magic, 4, 1, fp) != 1 || magic != CKPT_MAGIC ||
-            fread(hdr, 8, 2, fp) != 2 || hdr[1] != N ||
-            fread(f, sizeof(double) * 19, (size_t)N, fp) != (size_t)N) {
+        if (!f || !fp || fread(&h, sizeof h, 1, fp) != 1 || h.magic != CKPT_MAGIC || h.n_nodes != N) {
+            fprintf(stderr, ">>> cannot read checkpoint %s\n", restart);
+            exit(-5);
+        }
+        /* The stored elements are only meaningful in the storage they were
+         * written from: same precision, FP16 scale and shift. */
+        if (h.precision != si.precision || h.elem_bytes != si.elem_bytes ||
+            h.fp16_scale_log2 != si.fp16_scale_log2 || h.rho0 != si.rho0) {
+            fprintf(stderr,
+                    ">>> checkpoint %s holds precision %d (scale 2^%d, rho0 %g), this run %d (2^%d, %g)\n",
+                    restart, h.precision, h.fp16_scale_log2, h.rho0, si.precision, si.fp16_scale_log2,
+                    si.rho0);
+            exit(-5);
+        }
+        if (fread(f, (size_t)si.elem_bytes * 19, (size_t)N, fp) != (size_t)N) {
             fprintf(stderr, ">>> cannot read checkpoint %s\n", restart);
             exit(-5);
         }
         fclose(fp);
-        it0 = (int)hdr[0];
-        if (plb_write_populations(c, f)) exit(-5);
+        it0 = (int)h.step;
+        if (plb_write_state(c, f)) exit(-5);
         plb_host_free(f);
     }
 
@@ -351,16 +385,25 @@ int main(int argc, char *argv[])
 
     const char *ckpt = getenv("PLB_CHECKPOINT");
     if (ckpt && *ckpt) {
-        double *f = (double *)plb_host_alloc(sizeof(double) * 19 * (size_t)(N ? N : 1));
+        plb_state_info si;
+        plb_get_state_info(c, &si);
+        ckpt_header h;
+        memset(&h, 0, sizeof h);
+        h.magic = CKPT_MAGIC;
+        h.precision = si.precision;
+        h.elem_bytes = si.elem_bytes;
+        h.fp16_scale_log2 = si.fp16_scale_log2;
+        h.step = it; /* the step reached: it_max, or earlier on convergence */
+        h.n_nodes = N;
+        h.rho0 = si.rho0;
+        void *f = plb_host_alloc((size_t)si.elem_bytes * 19 * (size_t)(N ? N : 1));
         FILE *fp = fopen(ckpt, "wb");
-        uint32_t magic = CKPT_MAGIC;
-        int64_t hdr[2] = {it, N};  /* the step reached: it_max, or earlier on convergence */
-        if (!f || !fp || plb_read_populations(c, f)) {
+        if (!f || !fp || plb_read_state(c, f)) {
             fprintf(stderr, ">>> cannot write checkpoint %s\n", ckpt);
             exit(-3);
         }
-        int bad = fwrite(&magic, 4, 1, fp) != 1 || fwrite(hdr, 8, 2, fp) != 2 ||
-                  fwrite(f, sizeof(double) * 19, (size_t)N, fp) != (size_t)N;
+        int bad = fwrite(&h, sizeof h, 1, fp) != 1 ||
+                  fwrite(f, (size_t)si.elem_bytes * 19, (size_t)N, fp) != (size_t)N;
         bad |= fclose(fp) != 0;
         plb_host_free(f);
         if (bad) {

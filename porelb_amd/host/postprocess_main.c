@@ -156,6 +156,7 @@ int main(int argc, char *argv[])
     }
     int first = (mode == 'A' || mode == 'K') ? 2 : 3, n_slice = 0;
     if (first == 3) {
+        printf("argc = %d\n", argc); /* postprocess.c:107 */
         if (argc < 3) {
             print_help();
             exit(-2);

@@ -61,7 +61,8 @@ def test_ctypes_structs_match_the_header(plb, tmp_path):
     """The Python mirrors of the C structs have the header's size and field
     offsets (compiled and printed by gcc from include/plb.h and include/plb_tune.h)."""
     structs = {"plb_config": plb.Config, "plb_reduction": plb.Reduction, "plb_fault": plb.Fault,
-               "plb_sample": plb.Sample, "plb_tune_result": plb.TuneResult}
+               "plb_sample": plb.Sample, "plb_tune_result": plb.TuneResult,
+               "plb_state_info": plb.StateInfo}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "plb.h"', '#include "plb_tune.h"',
              "int main(void) {"]
     for cname, py in structs.items():

@@ -1,0 +1,90 @@
+"""The precision workload at its full size: a 512^3 periodic smoothed-noise
+domain (sigma 3, porosity 0.25, 33 554 432 pore nodes), tau 1.0, g_x 1e-6,
+half-way bounce-back.
+
+- FP64 macros in V order after 200 steps are bit-identical across the
+  int32 table in V order (SoA), TILE8 on pencils (AoSoA) and the BITMAP
+  index: the node order and index encoding change nothing.
+- FP64 half-way bounce-back conserves mass to 1e-12 (relative).
+- After 2000 steps FP32 storage (shifted populations, FP32 collide) stays
+  within 1e-4 of FP64 in K = nu q / g_x (porelb.c:89) and in the relative
+  L2 norm of the velocity field; FP16 storage with stochastic rounding is
+  held to its own stated bounds."""
+import hashlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (512, 512, 512)
+TAU, GX = 1.0, 1e-6
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def flag(plb):
+    f = plb.mask_noise(DIMS, 3.0, 0.25, 3)
+    assert int(f.sum()) == 33554432
+    return f
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+SHAPES = {
+    "i32-vorder-soa": dict(index_kind=0, layout=0, ordering=0),
+    "tile8-pencil-aosoa": dict(index_kind=1, layout=1, ordering=2, brick=(512, 64, 1), waves_per_simd=5),
+    "bitmap-pencil-aosoa": dict(index_kind=2, layout=1, ordering=2, brick=(512, 128, 1)),
+}
+
+
+@pytest.fixture(scope="module")
+def runs(plb, flag):
+    """FP64 to step 200 in every shape; then FP64 (TILE8 shape), FP32 and
+    FP16 to step 2000."""
+    out = {"sha200": {}}
+    for name, kw in SHAPES.items():
+        say("fp64 200 steps", name)
+        with plb.Lattice.from_flag(flag, tau=TAU, gx=GX, bb_mode=1, **kw) as lat:
+            m0 = lat.reduce()["mass"]
+            lat.step(0, 200)
+            out["sha200"][name] = sha(lat.read_macros())
+            out.setdefault("mass", {})[name] = (m0, lat.reduce()["mass"])
+            if name == "tile8-pencil-aosoa":
+                lat.step(200, 1800)
+                out[0] = (lat.reduce(), lat.read_macros()[1:].copy())
+    kw = SHAPES["tile8-pencil-aosoa"]
+    for prec, extra in ((1, {}), (2, dict(fp16_rounding=1))):
+        say("precision", prec, "2000 steps")
+        with plb.Lattice.from_flag(flag, tau=TAU, gx=GX, bb_mode=1, precision=prec, **kw, **extra) as lat:
+            lat.step(0, 2000)
+            out[prec] = (lat.reduce(), lat.read_macros()[1:].copy())
+    return out
+
+
+def test_fp64_bitwise_across_index_and_order(runs):
+    shas = set(runs["sha200"].values())
+    assert len(shas) == 1, runs["sha200"]
+
+
+def test_fp64_mass_conserved(runs):
+    for name, (m0, m1) in runs["mass"].items():
+        assert m0 == pytest.approx(33554432.0, rel=1e-13)
+        assert abs(m1 / m0 - 1) < 1e-12, name
+
+
+@pytest.mark.parametrize("prec,k_tol,u_tol", [(1, 1e-4, 1e-4), (2, 1e-2, 5e-2)])
+def test_reduced_precision_within_bounds(runs, prec, k_tol, u_tol):
+    r64, u64 = runs[0]
+    r, u = runs[prec]
+    k_err = abs(r["K"] / r64["K"] - 1)
+    u_err = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+    say(f"precision {prec}: K {r['K']:.10e} vs {r64['K']:.10e}, K err {k_err:.3e}, u L2 err {u_err:.3e}, "
+        f"mass drift {r['mass'] / r64['mass'] - 1:.3e}")
+    assert r64["K"] > 0
+    assert k_err <= k_tol and u_err <= u_tol
