@@ -163,7 +163,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipStreamSynchronize(c->stream);
     }
     void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
-                    c->inv_perm, c->node_xyz, c->macros, c->fault, c->red};
+                    c->inv_perm, c->key_perm, c->node_xyz, c->macros, c->fault, c->red};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -66,6 +66,15 @@ __global__ void k_batch_inv(uint32_t* inv, const uint32_t* sample_inv, int64_t n
     inv[off + g] = (uint32_t)(off + (sample_inv ? (int64_t)sample_inv[g] : g));
 }
 
+// key[off + (sample order of V-order node g)] = g: each node's V-order id
+// within its sample (keys the FP16 stochastic-rounding stream).
+__global__ void k_batch_key(uint32_t* key, const uint32_t* sample_inv, int64_t n, int64_t off)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    key[off + (sample_inv ? (int64_t)sample_inv[g] : g)] = (uint32_t)g;
+}
+
 static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* smp)
 {
     plb_ctx* c = b->c;
@@ -110,6 +119,8 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         if (e == hipSuccess) e = hipMemsetAsync(c->tab, 0, sizeof(int32_t) * (Q - 1) * total, c->stream);
         if (e == hipSuccess) e = hipMalloc(&c->node_xyz, sizeof(uint64_t) * total);
         if (e == hipSuccess) e = hipMalloc(&c->inv_perm, sizeof(uint32_t) * total);
+        if (e == hipSuccess) e = hipMalloc(&c->key_perm, sizeof(uint32_t) * total);
+        if (e == hipSuccess) e = hipMemsetAsync(c->key_perm, 0, sizeof(uint32_t) * total, c->stream);
         if (e != hipSuccess) rc = e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
     }
     for (int s = 0; s < b->S && !rc; s++) {
@@ -121,6 +132,8 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         if (!rc && b->n[s] > 0) {
             hipLaunchKernelGGL(k_batch_inv, dim3((unsigned)((b->n[s] + 255) / 256)), dim3(256), 0, c->stream,
                                c->inv_perm, invs[s], b->n[s], b->off[s]);
+            hipLaunchKernelGGL(k_batch_key, dim3((unsigned)((b->n[s] + 255) / 256)), dim3(256), 0, c->stream,
+                               c->key_perm, invs[s], b->n[s], b->off[s]);
             if (hipGetLastError() != hipSuccess) rc = PLB_ERR_HIP;
         }
     }

@@ -127,6 +127,7 @@ struct plb_ctx {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // G-step graphs by starting buffer
     double index_bytes = 72.0;  // pull-table bytes per node
     uint32_t* perm = nullptr;   // internal -> V order (null: identity)
+    uint32_t* key_perm = nullptr;  // batches: internal -> V order within the node's sample
     uint32_t* inv_perm = nullptr;
     uint64_t* node_xyz = nullptr;  // internal order
     double* macros = nullptr;   // [4][Npad]

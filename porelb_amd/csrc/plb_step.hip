@@ -51,6 +51,10 @@ struct StepArgs {
     // Graph replays: the step number is it + *it_base (set before each replay).
     const int64_t* it_base;
     int stochastic;  // FP16 stores: stochastic rounding
+    // Node id that keys the rounding stream: the V-order id of the node
+    // (within its sample for batches), so every node order and a batch give
+    // the same results as a single lattice in the V order.  Null: identity.
+    const uint32_t* key_perm;
 };
 
 __device__ __forceinline__ StepArgs with_step(const StepArgs& a)
@@ -316,7 +320,7 @@ __device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, float s
 
 template <typename S, int LAYOUT>
 __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, const int32_t t[Q],
-                                              const S raw[Q], unsigned long long fk)
+                                              const S raw[Q], unsigned long long fk, uint32_t key)
 {
     S* __restrict__ fout = (S*)a.fout;
     const float inv = a.inv_scale;
@@ -361,7 +365,7 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     S* __restrict__ onode = fout + fpos<LAYOUT>(0, i, a.Npad);
     uint32_t rnd = 0;
     if constexpr (sizeof(S) == 2)
-        if (a.stochastic) rnd = mix32((uint32_t)i ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
+        if (a.stochastic) rnd = mix32(key ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
 #pragma unroll
     for (int j = 0; j < Q; j++) {
         float eu = 0.f;
@@ -385,14 +389,23 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     }
 }
 
+// Key of node i's stochastic-rounding stream (FP16 storage only).
+template <typename S>
+__device__ __forceinline__ uint32_t sr_key(const StepArgs& a, int64_t i)
+{
+    if constexpr (sizeof(S) == 2)
+        if (a.stochastic && a.key_perm) return a.key_perm[i];
+    return (uint32_t)i;
+}
+
 template <typename S, int LAYOUT>
 __device__ __forceinline__ void collide_any(const StepArgs& a, int64_t i, const int32_t t[Q], S raw[Q],
-                                            unsigned long long fk)
+                                            unsigned long long fk, uint32_t key)
 {
     if constexpr (sizeof(S) == 8)
         collide_f64<LAYOUT>(a, i, t, (double*)raw, fk);
     else
-        collide_shift<S, LAYOUT>(a, i, t, raw, fk);
+        collide_shift<S, LAYOUT>(a, i, t, raw, fk, key);
 }
 
 template <typename S, int LAYOUT, int IDX>
@@ -400,10 +413,11 @@ __device__ __forceinline__ void node_any(const StepArgs& a, int64_t i, unsigned 
 {
     Entries<IDX> e;
     issue_entries<LAYOUT, IDX>(a, i, e);
+    const uint32_t key = sr_key<S>(a, i);
     decode_entries<IDX>(a, i, e);
     S raw[Q];
     gather<S, LAYOUT>(a, i, e.t, raw);
-    collide_any<S, LAYOUT>(a, i, e.t, raw, fk);
+    collide_any<S, LAYOUT>(a, i, e.t, raw, fk, key);
 }
 
 // Single lattice, one node per thread, XCD-contiguous block order.
@@ -435,11 +449,12 @@ __global__ void __launch_bounds__(512) k_step2(const StepArgs a0)
     if (two) issue_entries<LAYOUT, IDX>(a, i1, e1);
     decode_entries<IDX>(a, i0, e0);
     if (two) decode_entries<IDX>(a, i1, e1);
+    const uint32_t k0 = sr_key<S>(a, i0), k1 = two ? sr_key<S>(a, i1) : 0u;
     S r0[Q], r1[Q];
     gather<S, LAYOUT>(a, i0, e0.t, r0);
     if (two) gather<S, LAYOUT>(a, i1, e1.t, r1);
-    collide_any<S, LAYOUT>(a, i0, e0.t, r0, fk);
-    if (two) collide_any<S, LAYOUT>(a, i1, e1.t, r1, fk);
+    collide_any<S, LAYOUT>(a, i0, e0.t, r0, fk, k0);
+    if (two) collide_any<S, LAYOUT>(a, i1, e1.t, r1, fk, k1);
 }
 
 // Batch of samples: the grid enumerates the blocks of active samples
@@ -920,6 +935,7 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.div_nz = make_fastdiv(a.nz);
     a.it_base = nullptr;
     a.stochastic = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC;
+    a.key_perm = c->key_perm ? c->key_perm : c->perm;
     return a;
 }
 
