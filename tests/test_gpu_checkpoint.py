@@ -70,12 +70,15 @@ def test_cli_resume_equals_oracle_straight_run(tmp_path, num_p):
     assert r.returncode == 0, r.stderr
     V, start = O.build_V(flag, num_p)
     f0, f1 = O.init_equilibrium(len(V))
+    # A save at step t inside the loop holds the state after step t, i.e.
+    # after t + 1 steps (porelb.c:436); the final save after the loop holds
+    # the state after it_max steps (porelb.c:440).
     it = 0
-    for t in (10, 20, 23, 30, 40, 47):
-        mac = O.push_run(V, start, f0, f1, t - it, tau, gx, it0=it)
-        it = t
-        assert np.array_equal(read_parts(tmp_path, num_p, t), mac), t
-        if t == 23:
+    for name, done in ((10, 11), (20, 21), (23, 23), (30, 31), (40, 41), (47, 47)):
+        mac = O.push_run(V, start, f0, f1, done - it, tau, gx, it0=it)
+        it = done
+        assert np.array_equal(read_parts(tmp_path, num_p, name), mac), name
+        if name == 23:
             ck = read_ckpt(tmp_path / "ck1.bin", len(V), np.float64)
             assert ck["step"] == 23 and ck["precision"] == 0 and ck["elem"] == 8
             assert np.array_equal(ck["state"], f0)
