@@ -149,18 +149,22 @@ def cpu_baseline(P, w, max_seconds=20.0):
             "host_cores": host, "kind": "port", "sample": f"{dims[0]}^3 domain, {len(V)} pore nodes, {steps} FP64 steps, 1 thread"}
 
 
-def study_precision(P, max_steps, check, tol):
+PRECISION_VARIANTS = (("fp64", 0, 0), ("fp32_store", 1, 0), ("fp16_store_sr", 2, 1), ("fp16_store_rn", 2, 0))
+
+
+def study_precision(P, max_steps, check, tol, variants=PRECISION_VARIANTS):
     """Config 3: 512^3 smoothed noise (sigma 3, porosity 0.25), tau 1.0,
-    g_x 1e-6, half-way bounce-back; FP64 vs FP32-storage vs FP16-storage run
-    to steady state (|d<u_x>| <= tol |<u_x>| between checks).  Reports the
+    g_x 1e-6, half-way bounce-back; FP64 vs FP32-storage vs FP16-storage
+    (stochastic and round-to-nearest stores) run to steady state
+    (|d<u_x>| <= tol |<u_x>| between checks, or max_steps).  Reports the
     relative L2 error of the velocity field and of K against FP64."""
     w = WORKLOADS["noise512-fp32"]
     flag = make_mask(P, w)
     log(f"mask {w['dims']} porosity {flag.mean():.4f}")
     runs = {}
-    for prec in (0, 1, 2):
+    for name, prec, rnd in variants:
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
-                               ordering=2, brick=w["brick"], block_size=256,
+                               ordering=2, brick=w["brick"], block_size=256, fp16_rounding=rnd,
                                waves_per_simd=0 if prec == 2 else 5)  # the cap slows FP16 only
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
@@ -171,26 +175,26 @@ def study_precision(P, max_steps, check, tol):
                 it += check
                 r = lat.reduce()
                 if (it // check) % 20 == 0:
-                    log(f"precision {prec}: step {it}, <u_x> {r['mean_ux']:.12e}")
+                    log(f"{name}: step {it}, <u_x> {r['mean_ux']:.12e}")
                 if prev is not None and abs(r["mean_ux"] - prev) <= tol * abs(r["mean_ux"]):
                     break
                 prev = r["mean_ux"]
             wall = time.perf_counter() - t0
             mac = lat.read_macros()
-            runs[prec] = dict(steps=it, K=r["K"], mean_ux=r["mean_ux"], mass_drift=(r["mass"] - m0) / m0,
+            runs[name] = dict(steps=it, K=r["K"], mean_ux=r["mean_ux"], mass_drift=(r["mass"] - m0) / m0,
                               wall_s=wall, mlups=lat.N * it / wall / 1e6, u=mac[1:].copy(), nodes=lat.N,
                               converged=it < max_steps)
-        log(f"precision {prec}: {runs[prec]['steps']} steps, K {runs[prec]['K']:.9e}")
-    u64 = runs[0]["u"]
-    out = {"study": "precision", "workload": w["label"], "pore_nodes": runs[0]["nodes"], "tol": tol,
-           "check_every": check, "results": {}}
-    for prec, name in ((0, "fp64"), (1, "fp32_store"), (2, "fp16_store")):
-        r = runs[prec]
+        log(f"{name}: {runs[name]['steps']} steps, K {runs[name]['K']:.9e}")
+    ref = runs["fp64"]
+    out = {"study": "precision", "workload": w["label"], "pore_nodes": ref["nodes"], "tol": tol,
+           "check_every": check, "max_steps": max_steps, "results": {}}
+    for name, _, _ in variants:
+        r = runs[name]
         out["results"][name] = {
             "steps": r["steps"], "converged": r["converged"], "K": r["K"], "mean_ux": r["mean_ux"],
             "mass_rel_drift": r["mass_drift"], "mlups_incl_checks": round(r["mlups"], 1),
-            "velocity_rel_l2_err": float(np.linalg.norm(r["u"] - u64) / np.linalg.norm(u64)),
-            "K_rel_err": abs(r["K"] / runs[0]["K"] - 1.0)}
+            "velocity_rel_l2_err": float(np.linalg.norm(r["u"] - ref["u"]) / np.linalg.norm(ref["u"])),
+            "K_rel_err": abs(r["K"] / ref["K"] - 1.0)}
     print(json.dumps(out), flush=True)
 
 
@@ -373,11 +377,16 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
     kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256,
               waves_per_simd=waves)
-    b = P.Batch(flags, taus, gxs, **kw)
-    steps, upd, ms = b.run(max_steps, check, tol)
-    red = b.reduce()
-    nodes = list(b.nodes)
-    b.close()
+    modes = {}
+    for persistent in (False, True):
+        b = P.Batch(flags, taus, gxs, **kw)
+        steps, upd, ms = b.run(max_steps, check, tol, persistent=persistent)
+        red = b.reduce()
+        nodes = list(b.nodes)
+        b.close()
+        modes["persistent" if persistent else "per_step"] = dict(steps=[int(x) for x in steps], upd=int(upd), ms=ms,
+                                                                K=[r["K"] for r in red])
+        log(f"batch {'persistent' if persistent else 'per-step'}: {ms:.1f} ms, {upd / (ms * 1e-3) / 1e6:.0f} MLUPS")
     seq_upd, seq_ms, seq_steps = 0, 0.0, []
     for k, f in enumerate(flags):
         one = P.Batch([f], [taus[k]], [gxs[k]], **kw)
@@ -386,15 +395,22 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
         seq_ms += m
         seq_steps.append(int(st[0]))
         one.close()
+    log(f"batch sequential: {seq_ms:.1f} ms")
+
+    def rate(u, m):
+        return {"updates": int(u), "ms": round(m, 1), "mlups": round(u / (m * 1e-3) / 1e6, 1)}
+
+    pst, per = modes["persistent"], modes["per_step"]
     out = {"study": "batch", "samples": n_samples, "pore_nodes_total": int(sum(nodes)), "waves_per_simd": waves,
            "tol": tol, "check_every": check, "max_steps": max_steps,
-           "batched": {"updates": int(upd), "ms": round(ms, 1), "mlups": round(upd / (ms * 1e-3) / 1e6, 1)},
-           "sequential": {"updates": int(seq_upd), "ms": round(seq_ms, 1),
-                          "mlups": round(seq_upd / (seq_ms * 1e-3) / 1e6, 1)},
-           "steps_min": int(min(steps)), "steps_max": int(max(steps)),
-           "same_steps_as_sequential": [int(a) for a in steps] == seq_steps,
-           "K_range": [float(min(r["K"] for r in red)), float(max(r["K"] for r in red))]}
-    out["speedup"] = round(out["batched"]["mlups"] / out["sequential"]["mlups"], 3)
+           "per_step": rate(per["upd"], per["ms"]), "persistent": rate(pst["upd"], pst["ms"]),
+           "sequential": rate(seq_upd, seq_ms),
+           "steps_min": int(min(per["steps"])), "steps_max": int(max(per["steps"])),
+           "same_steps_all_three": per["steps"] == pst["steps"] == seq_steps,
+           "same_K_persistent_per_step": per["K"] == pst["K"],
+           "K_range": [float(min(per["K"])), float(max(per["K"]))]}
+    out["speedup_per_step_vs_sequential"] = round(out["per_step"]["mlups"] / out["sequential"]["mlups"], 3)
+    out["speedup_persistent_vs_sequential"] = round(out["persistent"]["mlups"] / out["sequential"]["mlups"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -96,3 +96,24 @@ def test_batch_convergence_steps_equal_sequential(plb, flags):
         finally:
             one.close()
         assert st[0] == steps[s], s
+
+
+def test_persistent_launch_equals_per_step_driver(plb, flags):
+    """The whole batch as one persistent launch: the same stop step, the same
+    reductions and the same V-order macros for every sample as one launch
+    per step."""
+    n = len(flags)
+    res = []
+    for persistent in (False, True):
+        b = plb.Batch(flags, [TAU] * n, [GX] * n, **SHAPE)
+        try:
+            steps, upd, ms = b.run(20000, 100, 1e-3, persistent=persistent)
+            say("persistent" if persistent else "per-step", f"{ms:.1f} ms", f"{upd / (ms * 1e-3) / 1e6:.0f} MLUPS")
+            res.append((steps, upd, b.reduce(), [b.read_macros(s) for s in range(n)]))
+        finally:
+            b.close()
+    (s0, u0, r0, m0), (s1, u1, r1, m1) = res
+    assert list(s0) == list(s1) and u0 == u1
+    for s in range(n):
+        assert r0[s] == r1[s], s
+        assert np.array_equal(m0[s], m1[s]), s
