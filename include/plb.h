@@ -284,7 +284,9 @@ int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double t
  * round trip per check).  The per-sample sums use the same fixed order in
  * both drivers, so both stop every sample at the same step and leave the
  * same states.  A barrier wait longer than 2 s ends the launch and returns
- * PLB_ERR_HIP. */
+ * PLB_ERR_HIP.  At most PLB_PERSIST_MAX_SAMPLES samples (PLB_ERR_INVALID
+ * beyond). */
+#define PLB_PERSIST_MAX_SAMPLES 2048
 int plb_batch_run_persistent(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
                              int64_t *steps_done, int64_t *updates, double *ms);
 /* Per-sample reductions after a run (n_samples entries). */

@@ -342,6 +342,7 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
                                         int64_t* steps_done, int64_t* updates, double* ms)
 {
     if (!b || max_steps < 0 || check_every < 1 || !(tol >= 0.0)) return PLB_ERR_INVALID;
+    if (b->S > PLB_PERSIST_MAX_SAMPLES) return PLB_ERR_INVALID;  // running flags live in LDS
     Range range("plb_batch_run_persistent");
     plb_ctx* c = b->c;
     PLB_HIP(hipSetDevice(c->device));
@@ -367,14 +368,14 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     PLB_HIP(d_steps.alloc(S));
     PLB_HIP(d_prev.alloc(S));
     PLB_HIP(d_flags.alloc(2));
-    PLB_HIP(d_bar.alloc(1));
+    PLB_HIP(d_bar.alloc(17 * 16));
     const int flags[2] = {n_active, 0};
     PLB_HIP(hipMemcpyAsync(d_act, act.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_cur, cur.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_steps, steps.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_prev, prev.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_flags, flags, sizeof flags, hipMemcpyHostToDevice, c->stream));
-    PLB_HIP(hipMemsetAsync(d_bar, 0, sizeof(unsigned long long), c->stream));
+    PLB_HIP(hipMemsetAsync(d_bar, 0, sizeof(unsigned long long) * 17 * 16, c->stream));
     PersistArgs p;
     p.f[0] = c->f[0];
     p.f[1] = c->f[1];
@@ -400,7 +401,8 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     p.abort = d_flags.p + 1;
     int clock_khz = 0;
     PLB_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device));
-    p.wait_ticks = (uint64_t)(clock_khz > 0 ? clock_khz : 100000) * 2000ull;  // 2 s of wall_clock64
+    if (clock_khz < 1000) clock_khz = 100000;  // no usable rate reported: the 100 MHz of gfx9
+    p.wait_ticks = (uint64_t)clock_khz * 2000ull;  // 2 s of wall_clock64
     int blocks = 0;
     PLB_HIP(hipEventRecord(c->ev0, c->stream));
     if (n_active > 0 && max_steps > 0) {
@@ -418,7 +420,9 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     PLB_HIP(hipMemcpy(cur.data(), d_cur, sizeof(int) * S, hipMemcpyDeviceToHost));
     PLB_HIP(hipMemcpy(steps.data(), d_steps, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
     if (hflags[1]) {
-        fprintf(stderr, "libplb: persistent batch run: a grid barrier wait exceeded 2 s (%d blocks)\n", blocks);
+        fprintf(stderr,
+                "libplb: persistent batch run: a grid barrier wait exceeded 2 s (%d blocks, wall clock %d kHz)\n",
+                blocks, clock_khz);
         return PLB_ERR_HIP;
     }
     if (key != NO_FAULT) return PLB_ERR_DENSITY;

@@ -208,7 +208,7 @@ struct PersistArgs {
     int64_t* steps;               // per sample: steps run
     double* prev;                 // per sample: <u_x> at the previous check (NaN before)
     double* partial;              // per chunk: u_x sum on check steps
-    unsigned long long* bar;      // arrivals at grid barriers (monotonic)
+    unsigned long long* bar;      // grid-barrier counters: 17 lines of 16 words, zeroed
     int* abort;                   // a barrier wait timed out
     int* n_active;                // samples still running
     uint64_t wait_ticks;          // wall_clock64 ticks a barrier wait may take

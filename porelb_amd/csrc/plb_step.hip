@@ -519,23 +519,46 @@ __device__ __forceinline__ double wave_sum_chunks(const double* partial, int64_t
 // applies the convergence rule and retires the sample; a second barrier
 // publishes the decisions.
 //
-// Barriers use agent-scope release/acquire fences around a monotonic 64-bit
-// arrival counter.  A wait is bounded by wall time (wall_clock64 ticks):
-// a work-group that waits longer sets `abort`, every work-group leaves the
-// loop at its next barrier, and the host reports the failure, so the kernel
-// always ends.
+// Barriers follow the cooperative grid-sync pattern: a block barrier, then
+// one thread per block releases the block's stores at agent scope and
+// arrives; after the whole grid has arrived it acquires, and a second block
+// barrier lets the block go on.  Arrivals and waits are spread over eight
+// groups of blocks (block id mod 8, the XCD a block is dispatched to), each
+// on its own 128-byte line: the last arrival of a group carries the group
+// to a grid-wide counter, and the last group releases every group's flag.
+// All counters are monotonic (barrier `gen` completes at gen x members), so
+// nothing is reset between barriers.  A wait is bounded by wall time
+// (wall_clock64 ticks): a block that waits longer sets `abort`, every block
+// leaves the loop at its next barrier, and the host reports the failure, so
+// the kernel always ends.  The samples' running flags are copied to LDS
+// after every barrier that can change them, so the chunk loop reads them
+// without a memory round trip.
+constexpr int PERSIST_MAX_SAMPLES = PLB_PERSIST_MAX_SAMPLES;
+constexpr int BAR_STRIDE = 16;  // u64 words between counters: one 128-byte line each
 
-__device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long long& target)
+__device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long long& gen)
 {
     __shared__ int ok;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    target += gridDim.x;
+    gen++;
     if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x, g = blockIdx.x & 7u;
+        const unsigned long long members = (G - 1 - g) / 8 + 1, groups = G < 8 ? G : 8;
+        unsigned long long* cnt = p.bar + g * BAR_STRIDE;
+        unsigned long long* all = p.bar + 8 * BAR_STRIDE;
+        unsigned long long* rel = p.bar + 9 * BAR_STRIDE;
         int good = 1;
-        __hip_atomic_fetch_add(p.bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (__hip_atomic_fetch_add(cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == gen * members) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            if (__hip_atomic_fetch_add(all, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == gen * groups) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                for (unsigned k = 0; k < groups; k++)
+                    __hip_atomic_store(rel + k * BAR_STRIDE, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         const uint64_t t0 = wall_clock64();
-        while (__hip_atomic_load(p.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(rel + g * BAR_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen) {
             if (__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 good = 0;
                 break;
@@ -545,12 +568,12 @@ __device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long
                 good = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(8);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ok = good;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return ok;
 }
 
@@ -558,10 +581,29 @@ template <typename S, int LAYOUT, int IDX>
 __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const PersistArgs p)
 {
     __shared__ double lds[16];
-    const int64_t G = gridDim.x;
-    const int64_t me = xcd_block(blockIdx.x, G);  // each XCD: a contiguous run of chunks per round
-    unsigned long long target = 0;
+    // aprefix[s] = active chunks of the samples before s (so sample s runs
+    // iff aprefix[s + 1] > aprefix[s]); rebuilt after every decision barrier.
+    __shared__ int aprefix[PERSIST_MAX_SAMPLES + 1];
+    __shared__ int stop;
     const int bs = blockDim.x;
+    // The active chunks, in sample order, are split into eight contiguous
+    // stretches, one per XCD (block id mod 8); the blocks of a stretch take
+    // its chunks in turn, so the XCD sweeps through its stretch in order.
+    const unsigned G = gridDim.x, grp = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const int64_t grp_blocks = (G - 1 - grp) / 8 + 1;
+    unsigned long long gen = 0;
+    auto rebuild = [&]() {
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int k = 0; k < p.nsamples; k++) {
+                aprefix[k] = acc;
+                if (p.active[k]) acc += (int)((p.sample_n[k] + bs - 1) / bs);
+            }
+            aprefix[p.nsamples] = acc;
+        }
+        __syncthreads();
+    };
+    rebuild();
     for (int64_t t = 0; t < p.max_steps; t++) {
         const bool check = (t + 1) % p.check_every == 0 || t + 1 == p.max_steps;
         const int par = (p.cur0 + (int)(t & 1)) & 1;
@@ -569,10 +611,12 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
         a.fout = p.f[par ^ 1];
         a.it = t;
         a.write_macros = check ? 1 : 0;
-        for (int64_t chunk = me; chunk < p.nchunks; chunk += G) {
-            const int s = p.chunk_sample[chunk];
-            // flags other work-groups wrote before the last barrier: coherent loads
-            if (!__hip_atomic_load(p.active + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        const int64_t total = aprefix[p.nsamples];
+        const int64_t j0 = grp * total / 8, j1 = (grp + 1) * total / 8;
+        int s = 0;
+        for (int64_t j = j0 + slot; j < j1; j += grp_blocks) {
+            while (aprefix[s + 1] <= j) s++;   // uniform per block; j only grows
+            const int64_t chunk = p.sample_off[s] / bs + (j - aprefix[s]);
             const int64_t i = chunk * bs + threadIdx.x;
             const bool live = i < p.sample_end[s];
             if (live) {
@@ -585,33 +629,40 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
                 if (threadIdx.x == 0) p.partial[chunk] = t_sum;
             }
         }
-        if (!grid_barrier(p, target)) return;
+        if (!grid_barrier(p, gen)) return;
         // Density guard: stop after the faulting step.  Only a fault of this
-        // step or earlier counts: a faster work-group may already be in the
-        // next step, and every work-group must leave at the same barrier.
-        const unsigned long long fk = __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (fk != NO_FAULT && (int64_t)(fk >> 32) <= t) return;
+        // step or earlier counts: a faster block may already be in the next
+        // step, and every block must leave at the same barrier.
+        if (threadIdx.x == 0) {
+            const unsigned long long fk = __hip_atomic_load(a.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            stop = fk != NO_FAULT && (int64_t)(fk >> 32) <= t;
+        }
+        __syncthreads();
+        if (stop) return;
         if (!check) continue;
         if (threadIdx.x < 64) {
-            for (int64_t s = me; s < p.nsamples; s += G) {
-                if (!__hip_atomic_load(p.active + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-                const double sum = wave_sum_chunks(p.partial, p.sample_off[s] / bs, (p.sample_n[s] + bs - 1) / bs);
+            for (int64_t q = xcd_block(blockIdx.x, G); q < p.nsamples; q += G) {
+                if (aprefix[q + 1] == aprefix[q]) continue;   // not running
+                const double sum = wave_sum_chunks(p.partial, p.sample_off[q] / bs, (p.sample_n[q] + bs - 1) / bs);
                 if (threadIdx.x == 0) {
-                    const double mean = sum / (double)p.sample_n[s];
-                    const double scale = fmax(fabs(mean), 1e-3 * fabs(p.gx[s]));
-                    const double prev = p.prev[s];
-                    p.steps[s] = t + 1;
-                    p.cur[s] = par ^ 1;
+                    const double mean = sum / (double)p.sample_n[q];
+                    const double scale = fmax(fabs(mean), 1e-3 * fabs(p.gx[q]));
+                    const double prev = p.prev[q];
+                    p.steps[q] = t + 1;
+                    p.cur[q] = par ^ 1;
                     if (!isnan(prev) && fabs(mean - prev) <= p.tol * scale) {
-                        __hip_atomic_store(p.active + s, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        p.active[q] = 0;
                         __hip_atomic_fetch_add(p.n_active, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
-                    p.prev[s] = mean;
+                    p.prev[q] = mean;
                 }
             }
         }
-        if (!grid_barrier(p, target)) return;
-        if (__hip_atomic_load(p.n_active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+        if (!grid_barrier(p, gen)) return;
+        if (threadIdx.x == 0) stop = __hip_atomic_load(p.n_active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        __syncthreads();
+        if (stop) return;
+        rebuild();
     }
 }
 
@@ -1041,10 +1092,18 @@ static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int* blocks)
 {
     auto kern = k_batch_persistent<S, LAYOUT, IDX>;
     const int bs = c->cfg.block_size;
-    const unsigned lds = step_lds_bytes(c, bs);
+    // The wave cap's LDS share per block, less what the kernel holds itself.
+    hipFuncAttributes fa;
+    PLB_HIP(hipFuncGetAttributes(&fa, (const void*)kern));
+    unsigned lds = step_lds_bytes(c, bs);
+    lds = lds > fa.sharedSizeBytes ? (unsigned)((lds - fa.sharedSizeBytes) / 512 * 512) : 0u;
     int per_cu = 0, cus = 0;
     PLB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, bs, lds));
     PLB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    // One block per CU below the computed residency: a grid at exactly the
+    // computed limit waited at its first barrier for blocks that never
+    // started (the limit is an estimate; the bounded wait reported it).
+    if (per_cu > 1) per_cu--;
     const int64_t g = std::min<int64_t>((int64_t)per_cu * cus, p.nchunks);
     if (g < 1) return PLB_ERR_INVALID;
     StepArgs a = make_args(c, 0, false);
