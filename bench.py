@@ -43,7 +43,7 @@ WORKLOADS = {
     "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=2,
                              brick=(1024, 128, 1), block=256, waves=5),
     "spheres1024-fp16": dict(SPHERES1024, precision=2, bb=1, index=1, layout=1, ordering=2,
-                             brick=(1024, 64, 1), block=128),
+                             brick=(1024, 64, 1), block=128, fp16_rounding=1),
     "spheres1024-fp64": dict(SPHERES1024, precision=0, bb=0, index=0, layout=1, ordering=2,
                              brick=(1024, 64, 1), block=256),
     "spheres256-fp64": dict(kind="spheres", dims=(256, 256, 256), rmin=6.0, rmax=12.0, porosity=0.30,
@@ -87,10 +87,10 @@ def make_mask(P, w):
 
 
 def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256, index=0,
-              brick=(16, 4, 4), waves=0):
+              brick=(16, 4, 4), waves=0, fp16_rounding=1):
     cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=precision, bb_mode=bb, layout=layout,
                            ordering=ordering, block_size=block, index_kind=index, brick=brick,
-                           waves_per_simd=waves)
+                           waves_per_simd=waves, fp16_rounding=fp16_rounding)
     t0 = time.perf_counter()
     lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
     log(f"lattice built: {lat.N} pore nodes, precision {precision}, {time.perf_counter() - t0:.1f} s")
@@ -209,7 +209,7 @@ def precision_runs(P, flag, w, precs, steps, chunk=500):
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
                                ordering=2, brick=w["brick"], block_size=256,
                                waves_per_simd=0 if prec == 2 else w.get("waves", 5),
-                               fp16_rounding=w.get("fp16_rounding", 0))
+                               fp16_rounding=w.get("fp16_rounding", 1))
         with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
             m0 = lat.reduce()["mass"]
             ms, it = 0.0, 0
@@ -422,6 +422,7 @@ def main():
     ap.add_argument("--workload", default="spheres1024-fp32", choices=sorted(WORKLOADS))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-fp16", action="store_true")
     ap.add_argument("--layout", type=int, default=None)
     ap.add_argument("--ordering", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
@@ -479,7 +480,7 @@ def main():
     shape["waves"] = args.waves if args.waves is not None else w.get("waves", 0)
     main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, shape["layout"],
                          shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)),
-                         shape["waves"])
+                         shape["waves"], w.get("fp16_rounding", 1))
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
     avg_ms = main_run["event_ms"] / steps
@@ -519,22 +520,35 @@ def main():
                      "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads, "
                                          "at most 5 waves per SIMD"}
 
+    # FP16 storage (stochastic rounding) in its own fastest shape, beside the line.
+    if not args.skip_fp16 and w["precision"] == 1:
+        w16 = WORKLOADS["spheres1024-fp16"]
+        st16 = max(20, steps // 2)
+        r16 = time_path(P, flag, w, 2, 1, st16, max(5, args.warmup // 2), w16["layout"], w16["ordering"],
+                        w16["block"], w16["index"], w16["brick"], 0, w16["fp16_rounding"])
+        m16 = r16["nodes"] * st16 / r16["wall_s"] / 1e6
+        secondary.update({"fp16_store_mlups": round(m16, 1),
+                          "fp16_store_roofline_frac": round((2 * 19 * 2 + 19 * 4) * r16["nodes"] /
+                                                            (r16["event_ms"] / st16 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "fp16_store_shape": "stochastic rounding, TILE8, AoSoA, pencils of 64 rows, 128 threads"})
+
     # The metric's precision gate: the same steps from rest in FP64 and in the
-    # workload's storage (both half-way bounce-back), K = nu q / g_x
+    # reduced storage forms (both half-way bounce-back), K = nu q / g_x
     # (nu = (tau - 1/2)/3, porelb.c:89) and the velocity field compared.
     precision = None
     if args.gate_steps > 0 and w["precision"] != 0:
         t0 = time.perf_counter()
-        runs = precision_runs(P, flag, w, (0, w["precision"]), args.gate_steps)
-        err = errors_vs_fp64(runs, w["precision"])
-        precision = {
-            "compare": f"{PREC_NAME[w['precision']]} vs f64, both half-way bounce-back, {args.gate_steps} steps "
-                       "from rest on this workload",
-            "K_f64": runs[0]["K"], "K": runs[w["precision"]]["K"],
-            "K_rel_err": err["K_rel_err"], "gate_K_rel_err": 1e-4, "pass": err["K_rel_err"] <= 1e-4,
-            "velocity_rel_l2_err": err["velocity_rel_l2_err"],
-            "mass_rel_drift_f64": runs[0]["mass_drift"], "mass_rel_drift": runs[w["precision"]]["mass_drift"],
-            "wall_s": round(time.perf_counter() - t0, 1)}
+        precs = (0, 1, 2) if (w["precision"] == 1 and not args.skip_fp16) else (0, w["precision"])
+        runs = precision_runs(P, flag, w, precs, args.gate_steps)
+        precision = {"compare": f"stored precision vs f64, both half-way bounce-back, {args.gate_steps} steps "
+                                "from rest on this workload; FP16 with stochastic rounding",
+                     "K_f64": runs[0]["K"], "mass_rel_drift_f64": runs[0]["mass_drift"], "gate_K_rel_err": 1e-4}
+        for prec in precs[1:]:
+            err = errors_vs_fp64(runs, prec)
+            precision[{1: "f32", 2: "f16"}[prec]] = {
+                "K": runs[prec]["K"], "K_rel_err": err["K_rel_err"], "pass": err["K_rel_err"] <= 1e-4,
+                "velocity_rel_l2_err": err["velocity_rel_l2_err"], "mass_rel_drift": runs[prec]["mass_drift"]}
+        precision["wall_s"] = round(time.perf_counter() - t0, 1)
         del runs
 
     out = {

@@ -51,6 +51,7 @@ struct StepArgs {
     // Graph replays: the step number is it + *it_base (set before each replay).
     const int64_t* it_base;
     int stochastic;  // FP16 stores: stochastic rounding
+    int has_const;   // the table may hold constant entries (reference bounce order)
     // Node id that keys the rounding stream: the V-order id of the node
     // (within its sample for batches), so every node order and a batch give
     // the same results as a single lattice in the V order.  Null: identity.
@@ -281,16 +282,20 @@ __device__ __forceinline__ void collide_f64(const StepArgs& a, int64_t i, const 
 }
 
 // ---------------------------------------------------------------- shifted FP32 / FP16 storage
-__device__ __forceinline__ float to_dev(float v, float) { return v; }
-// fma(convert(h), inv, +0) is the rounded product, in one mixed-precision FMA
-__device__ __forceinline__ float to_dev(__half v, float inv) { return fmaf(__half2float(v), inv, 0.0f); }
-__device__ __forceinline__ void st_dev(float* p, int64_t k, float v, float)
+// The collision works in the stored scale: ds_j = d_j * sc (sc = 2^k for
+// FP16, 1 for FP32).  Scaling by a power of two commutes with every FP32
+// rounding, so the moments unscaled once and the stores with the scale
+// folded into the collision constants are bit-identical to unscaling each
+// population and scaling each store.
+__device__ __forceinline__ float to_float(float v) { return v; }
+__device__ __forceinline__ float to_float(__half v) { return __half2float(v); }
+__device__ __forceinline__ void st_dev(float* p, int64_t k, float v)
 {
     __builtin_nontemporal_store(v, p + k);
 }
-__device__ __forceinline__ void st_dev(__half* p, int64_t k, float v, float sc)
+__device__ __forceinline__ void st_dev(__half* p, int64_t k, float v)
 {
-    const unsigned short b = __half_as_ushort(__float2half_rn(v * sc));
+    const unsigned short b = __half_as_ushort(__float2half_rn(v));
     __builtin_nontemporal_store(b, (unsigned short*)p + k);
 }
 
@@ -312,9 +317,9 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
     x ^= x >> 16;
     return x;
 }
-__device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, float sc, uint32_t r)
+__device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, uint32_t r)
 {
-    const plb_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(plb_half2{0, 0}, v * sc, r, false);
+    const plb_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(plb_half2{0, 0}, v, r, false);
     __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, h[0]), (unsigned short*)p + k);
 }
 
@@ -323,21 +328,30 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
                                               const S raw[Q], unsigned long long fk, uint32_t key)
 {
     S* __restrict__ fout = (S*)a.fout;
-    const float inv = a.inv_scale;
-    float d[Q];
+    constexpr bool scaled = sizeof(S) == 2;
+    const float inv = a.inv_scale, sc = a.scale;
+    float ds[Q];
 #pragma unroll
-    for (int j = 0; j < Q; j++) d[j] = to_dev(raw[j], inv);
+    for (int j = 0; j < Q; j++) ds[j] = to_float(raw[j]);
+    if (a.has_const) {  // constant entries exist only under the reference bounce order
 #pragma unroll
-    for (int j = 1; j < Q; j++)
-        if (t[j] == TAB_CONST) d[j] = 0.f;
+        for (int j = 1; j < Q; j++)
+            if (t[j] == TAB_CONST) ds[j] = 0.f;
+    }
 
     float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
     for (int j = 0; j < Q; j++) {
-        drho += d[j];
-        if (EX[j]) jx += EX[j] > 0 ? d[j] : -d[j];
-        if (EY[j]) jy += EY[j] > 0 ? d[j] : -d[j];
-        if (EZ[j]) jz += EZ[j] > 0 ? d[j] : -d[j];
+        drho += ds[j];
+        if (EX[j]) jx += EX[j] > 0 ? ds[j] : -ds[j];
+        if (EY[j]) jy += EY[j] > 0 ? ds[j] : -ds[j];
+        if (EZ[j]) jz += EZ[j] > 0 ? ds[j] : -ds[j];
+    }
+    if constexpr (scaled) {
+        drho *= inv;
+        jx *= inv;
+        jy *= inv;
+        jz *= inv;
     }
     const float rho0 = (float)a.rho0;
     const float rho = rho0 + drho;
@@ -361,10 +375,13 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     // tolerance against FP64, not to bitwise equality.
     const float usq = 1.5f * fmaf(u, u, fmaf(v, v, w * w));
     const float fu = -c2 * u;  // force term c2 (e_jx - u) for e_jx = 0, then +-c2
-    const float sc = a.scale;
+    // the store scale folded into the constants of the dfeq and force terms
+    const float omega_s = scaled ? omega * sc : omega;
+    const float f0s = scaled ? fu * sc : fu, fps = scaled ? (fu + c2) * sc : fu + c2,
+                fms = scaled ? (fu - c2) * sc : fu - c2;
     S* __restrict__ onode = fout + fpos<LAYOUT>(0, i, a.Npad);
     uint32_t rnd = 0;
-    if constexpr (sizeof(S) == 2)
+    if constexpr (scaled)
         if (a.stochastic) rnd = mix32(key ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
 #pragma unroll
     for (int j = 0; j < Q; j++) {
@@ -376,15 +393,15 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
         const float wj = (float)TP[j];
         const float dfeq = wj * fmaf(rho, P, drho);
         const float feq = fmaf(wj, rho0, dfeq);
-        const float frc = EX[j] > 0 ? fu + c2 : (EX[j] < 0 ? fu - c2 : fu);
-        const float o = fmaf(c1, d[j], fmaf(omega, dfeq, frc * feq));
-        if constexpr (sizeof(S) == 2) {
+        const float frc = EX[j] > 0 ? fps : (EX[j] < 0 ? fms : f0s);
+        const float o = fmaf(c1, ds[j], fmaf(omega_s, dfeq, frc * feq));
+        if constexpr (scaled) {
             if (a.stochastic)
-                st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc, rnd + (uint32_t)j * 0x9e3779b9u);
+                st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, rnd + (uint32_t)j * 0x9e3779b9u);
             else
-                st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
+                st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
         } else {
-            st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o, sc);
+            st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
         }
     }
 }
@@ -987,6 +1004,7 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.it_base = nullptr;
     a.stochastic = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC;
     a.key_perm = c->key_perm ? c->key_perm : c->perm;
+    a.has_const = c->cfg.bb_mode == PLB_BB_REFCOMPAT;
     return a;
 }
 
