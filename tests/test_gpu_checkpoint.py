@@ -181,3 +181,28 @@ def test_duplicate_voxel_records_refused(plb, ordering):
     assert ei.value.code == -3
     with plb.Lattice.from_V(V, start, flag.shape) as lat:
         lat.step(0, 2)
+
+
+@pytest.mark.parametrize("ordering,brick", [(0, (16, 4, 4)), (1, (8, 4, 2)), (2, (28, 4, 1))])
+def test_fp16_stochastic_rounding_independent_of_node_order(plb, ordering, brick):
+    """The stochastic-rounding stream is keyed by the node's V-order id, so FP16
+    storage gives the same state in every node order and layout, and a batch
+    sample the same as its own lattice."""
+    flag = plb.mask_spheres((24, 20, 28), 2.0, 4.0, 0.45, 8)
+    kw = dict(tau=0.7, gx=2e-5, precision=2, bb_mode=1, fp16_rounding=1, fp16_scale_log2=20)
+    with plb.Lattice.from_flag(flag, **kw) as ref:
+        ref.step(0, 40)
+        want = ref.read_state()
+    with plb.Lattice.from_flag(flag, index_kind=1, layout=1, ordering=ordering, brick=brick, **kw) as lat:
+        lat.step(0, 40)
+        assert np.array_equal(lat.read_state(), want)
+    other = plb.mask_noise((20, 18, 22), 2.0, 0.4, 3)
+    b = plb.Batch([other, flag], [0.9, 0.7], [1e-5, 2e-5], precision=2, bb_mode=1, fp16_rounding=1,
+                  fp16_scale_log2=20, index_kind=1, layout=1, ordering=ordering, brick=brick)
+    try:
+        b.run(40, 40, 0.0)
+        with plb.Lattice.from_flag(flag, **kw) as ref:
+            ref.step(0, 40)
+            assert np.array_equal(b.read_macros(1), ref.read_macros())
+    finally:
+        b.close()

@@ -8,8 +8,8 @@ half-way bounce-back.
 - FP64 half-way bounce-back conserves mass to 1e-12 (relative).
 - After 2000 steps FP32 storage (shifted populations, FP32 collide) stays
   within 1e-4 of FP64 in K = nu q / g_x (porelb.c:89) and in the relative
-  L2 norm of the velocity field; FP16 storage with stochastic rounding is
-  held to its own stated bounds."""
+  L2 norm of the velocity field; FP16 storage with stochastic rounding
+  stays within 1e-4 in K and 5e-3 in the velocity field."""
 import hashlib
 
 import numpy as np
@@ -78,7 +78,9 @@ def test_fp64_mass_conserved(runs):
         assert abs(m1 / m0 - 1) < 1e-12, name
 
 
-@pytest.mark.parametrize("prec,k_tol,u_tol", [(1, 1e-4, 1e-4), (2, 1e-2, 5e-2)])
+# FP32 and FP16 (stochastic rounding) meet the metric's K gate (1e-4); FP16's
+# velocity field carries its rounding noise (8.5e-4 measured at 2000 steps).
+@pytest.mark.parametrize("prec,k_tol,u_tol", [(1, 1e-4, 1e-4), (2, 1e-4, 5e-3)])
 def test_reduced_precision_within_bounds(runs, prec, k_tol, u_tol):
     r64, u64 = runs[0]
     r, u = runs[prec]
