@@ -250,6 +250,17 @@ const char *plb_strerror(int code);
 void *plb_host_alloc(size_t bytes);
 void plb_host_free(void *p);
 
+/* plb_read_macros for the V-order nodes [first, first + n) without any
+ * device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The gather to V order is staged through
+ * a fixed device array of PLB_NOALLOC_MAX_NODES x 4 doubles (32 MiB) that each
+ * GPU the library runs on holds until the process ends; n beyond that
+ * returns PLB_ERR_RANGE.  Calls that overlap on one GPU race on that array. */
+#define PLB_NOALLOC_MAX_NODES (1 << 20)
+int plb_read_macros_noalloc(plb_ctx *c, int64_t first, int64_t n, double *rho, double *ux, double *uy,
+                            double *uz);
+
 /* ---- preprocess on the device (preprocess.c:29-164 build_V): flag ->
  * V records (V order) and V_sub_start.  Call with V_out == NULL to get the
  * pore count only.  Fields preprocess.c leaves uninitialised (SOLID pid/id,

@@ -31,6 +31,7 @@ LAYOUT_SOA, LAYOUT_AOSOA = 0, 1
 ORDER_REFERENCE, ORDER_BRICK, ORDER_PENCIL = 0, 1, 2
 INDEX_I32, INDEX_TILE8, INDEX_BITMAP = 0, 1, 2
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+NOALLOC_MAX_NODES = 1 << 20     # PLB_NOALLOC_MAX_NODES
 TYPE_FLUID, TYPE_SOLID, TYPE_EXTERN = 0, 1, 2
 ERR_DENSITY = -6
 
@@ -105,6 +106,7 @@ def _lib():
             "plb_read_populations": ([p, p], i32),
             "plb_write_populations": ([p, p], i32),
             "plb_read_table": ([p, p], i32),
+            "plb_read_macros_noalloc": ([p, i64, i64, p, p, p, p], i32),
             "plb_get_state_info": ([p, p], i32),
             "plb_read_state": ([p, p], i32),
             "plb_write_state": ([p, p], i32),

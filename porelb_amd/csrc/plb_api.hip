@@ -131,6 +131,8 @@ int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, in
     PLB_HIP(hipSetDevice(c->device));
     PLB_HIP(hipDeviceGetAttribute(&c->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                                   c->device));
+    const int rc = noalloc_setup(c->device);  // the no-alloc read-back's staging array
+    if (rc) return rc;
     PLB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PLB_HIP(hipEventCreate(&c->ev0));
     PLB_HIP(hipEventCreate(&c->ev1));

@@ -249,5 +249,6 @@ int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, cons
 int permute_ref_table(const int32_t* tabref, int64_t n, const uint32_t* inv_perm, int32_t* out,
                       hipStream_t s);
 int ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, int num_p);
+int noalloc_setup(int device);
 int ctx_alloc_state(plb_ctx* c);
 }  // namespace plb
