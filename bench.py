@@ -87,7 +87,10 @@ def make_mask(P, w):
 
 
 def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256, index=0,
-              brick=(16, 4, 4), waves=0, fp16_rounding=1):
+              brick=(16, 4, 4), waves=0, fp16_rounding=1, keep=False):
+    """Times `steps` steps after `warmup` on a new lattice.  keep=True returns
+    the lattice open (result["lattice"]) for the precision gate, so the gate's
+    steps run on the allocation that was timed."""
     cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=precision, bb_mode=bb, layout=layout,
                            ordering=ordering, block_size=block, index_kind=index, brick=brick,
                            waves_per_simd=waves, fp16_rounding=fp16_rounding)
@@ -102,9 +105,15 @@ def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, bl
         red = lat.reduce()
         bpu = lat.bytes_per_update()
         n = lat.N
-    finally:
+    except BaseException:
         lat.close()
-    return dict(nodes=n, wall_s=wall, event_ms=ev_ms, bytes_per_update=bpu, reduction=red)
+        raise
+    out = dict(nodes=n, wall_s=wall, event_ms=ev_ms, bytes_per_update=bpu, reduction=red)
+    if keep:
+        out["lattice"] = lat
+    else:
+        lat.close()
+    return out
 
 
 def cpu_baseline(P, w, max_seconds=20.0):
@@ -198,19 +207,27 @@ def study_precision(P, max_steps, check, tol, variants=PRECISION_VARIANTS):
     print(json.dumps(out), flush=True)
 
 
-def precision_runs(P, flag, w, precs, steps, chunk=500):
+def precision_runs(P, flag, w, precs, steps, chunk=500, lattices=None):
     """The same number of steps from rest in each storage precision of
     `precs` (the first one is the FP64 reference), all with half-way
-    bounce-back on the workload's TILE8 / pencil shape.  Returns per precision
-    the FP64 reductions after the last step, the mass drift, the velocity
-    field (V order) and the MLUPS of the timed steps."""
+    bounce-back.  A precision found in `lattices` (an open half-way lattice of
+    that precision) is restarted from rest on it; the others run on a new
+    lattice in the workload's TILE8 / pencil shape.  Launch shapes do not
+    change results.  Returns per precision the FP64 reductions after the last
+    step, the mass drift, the velocity field (V order) and the MLUPS of the
+    timed steps."""
+    import contextlib
+
     runs = {}
     for prec in precs:
         cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=prec, bb_mode=1, index_kind=1, layout=1,
                                ordering=2, brick=w["brick"], block_size=256,
                                waves_per_simd=0 if prec == 2 else w.get("waves", 5),
                                fp16_rounding=w.get("fp16_rounding", 1))
-        with P.Lattice.from_flag(flag, 1, cfg=cfg) as lat:
+        given = (lattices or {}).get(prec)
+        if given is not None:
+            given.init_equilibrium()
+        with (contextlib.nullcontext(given) if given is not None else P.Lattice.from_flag(flag, 1, cfg=cfg)) as lat:
             m0 = lat.reduce()["mass"]
             ms, it = 0.0, 0
             while it < steps:
@@ -478,9 +495,13 @@ def main():
     shape = {k: (getattr(args, k) if getattr(args, k) is not None else w[k])
              for k in ("layout", "ordering", "block", "index")}
     shape["waves"] = args.waves if args.waves is not None else w.get("waves", 0)
+    # The timed lattice stays open: the precision gate below restarts it from
+    # rest, so every launch of this kernel in the run is on one allocation.
+    gate = args.gate_steps > 0 and w["precision"] != 0 and w["bb"] == 1
     main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, shape["layout"],
                          shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)),
-                         shape["waves"], w.get("fp16_rounding", 1))
+                         shape["waves"], w.get("fp16_rounding", 1), keep=gate)
+    kept = {w["precision"]: main_run.pop("lattice")} if gate else {}
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
     avg_ms = main_run["event_ms"] / steps
@@ -525,7 +546,9 @@ def main():
         w16 = WORKLOADS["spheres1024-fp16"]
         st16 = max(20, steps // 2)
         r16 = time_path(P, flag, w, 2, 1, st16, max(5, args.warmup // 2), w16["layout"], w16["ordering"],
-                        w16["block"], w16["index"], w16["brick"], 0, w16["fp16_rounding"])
+                        w16["block"], w16["index"], w16["brick"], 0, w16["fp16_rounding"], keep=gate)
+        if gate:
+            kept[2] = r16.pop("lattice")
         m16 = r16["nodes"] * st16 / r16["wall_s"] / 1e6
         secondary.update({"fp16_store_mlups": round(m16, 1),
                           "fp16_store_roofline_frac": round((2 * 19 * 2 + 19 * 4) * r16["nodes"] /
@@ -536,10 +559,10 @@ def main():
     # reduced storage forms (both half-way bounce-back), K = nu q / g_x
     # (nu = (tau - 1/2)/3, porelb.c:89) and the velocity field compared.
     precision = None
-    if args.gate_steps > 0 and w["precision"] != 0:
+    if gate:
         t0 = time.perf_counter()
         precs = (0, 1, 2) if (w["precision"] == 1 and not args.skip_fp16) else (0, w["precision"])
-        runs = precision_runs(P, flag, w, precs, args.gate_steps)
+        runs = precision_runs(P, flag, w, precs, args.gate_steps, lattices=kept)
         precision = {"compare": f"stored precision vs f64, both half-way bounce-back, {args.gate_steps} steps "
                                 "from rest on this workload; FP16 with stochastic rounding",
                      "K_f64": runs[0]["K"], "mass_rel_drift_f64": runs[0]["mass_drift"], "gate_K_rel_err": 1e-4}
@@ -550,6 +573,8 @@ def main():
                 "velocity_rel_l2_err": err["velocity_rel_l2_err"], "mass_rel_drift": runs[prec]["mass_drift"]}
         precision["wall_s"] = round(time.perf_counter() - t0, 1)
         del runs
+    for lat in kept.values():
+        lat.close()
 
     out = {
         "metric": "pore-node MLUPS",

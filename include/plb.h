@@ -308,6 +308,11 @@ int plb_batch_reduce(plb_batch *b, plb_reduction *out);
  * (porelb.c:375-391), per sample.  Any pointer may be NULL. */
 int plb_batch_read_macros(plb_batch *b, int sample, double *rho, double *ux, double *uy, double *uz);
 int64_t plb_batch_num_nodes(const plb_batch *b, int sample);
+/* After a run returned PLB_ERR_DENSITY: the sample whose node tripped the
+ * density guard first (*sample) and the fault as plb_last_fault gives it,
+ * with node and local_id the V-order id within that sample and pid 0.
+ * *sample = -1 and out->tripped = 0 if no guard tripped. */
+int plb_batch_last_fault(plb_batch *b, int *sample, plb_fault *out);
 void plb_batch_destroy(plb_batch *b);
 
 /* ---- synthetic voxel masks (inputs for tests and benchmarks; the reference

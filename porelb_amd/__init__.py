@@ -125,6 +125,7 @@ def _lib():
             "plb_run_to_convergence": ([p, i64, i64, i64, d, p, p], i32),
             "plb_batch_read_macros": ([p, i32, p, p, p, p], i32),
             "plb_batch_num_nodes": ([p, i32], i64),
+            "plb_batch_last_fault": ([p, p, p], i32),
             "plb_batch_destroy": ([p], None),
             "plb_mask_noise": ([p, i32, i32, i32, d, d, C.c_uint64], i32),
             "plb_mask_spheres": ([p, i32, i32, i32, d, d, d, C.c_uint64], i32),
@@ -422,6 +423,13 @@ class Batch:
         arr = (Reduction * self.n)()
         _check(_lib().plb_batch_reduce(self._h, arr), "plb_batch_reduce")
         return [arr[s].as_dict() for s in range(self.n)]
+
+    def last_fault(self):
+        """(sample, fault dict) of the first density-guard trip (sample -1: none)."""
+        s = C.c_int(-1)
+        f = Fault()
+        _check(_lib().plb_batch_last_fault(self._h, C.byref(s), C.byref(f)), "plb_batch_last_fault")
+        return s.value, {k: getattr(f, k) for k, _ in f._fields_}
 
     def read_macros(self, sample):
         """(4, n) rho, ux, uy, uz of one sample in its V order."""

@@ -442,6 +442,40 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     return PLB_OK;
 }
 
+extern "C" int plb_batch_last_fault(plb_batch* b, int* sample, plb_fault* out)
+{
+    if (!b || !sample || !out) return PLB_ERR_INVALID;
+    plb_ctx* c = b->c;
+    PLB_HIP(hipSetDevice(c->device));
+    memset(out, 0, sizeof *out);
+    *sample = -1;
+    unsigned long long key = NO_FAULT;
+    PLB_HIP(hipMemcpy(&key, c->fault, sizeof key, hipMemcpyDeviceToHost));
+    if (key == NO_FAULT) return PLB_OK;
+    // the key holds the internal (concatenated) index of the node
+    const int64_t i = (int64_t)(key & 0xffffffffull);
+    int s = -1;
+    for (int q = 0; q < b->S; q++)
+        if (i >= b->off[q] && i < b->off[q] + b->n[q]) s = q;
+    if (s < 0) return PLB_ERR_BOUNDS;
+    uint32_t g = 0;  // V-order id within the sample
+    PLB_HIP(hipMemcpy(&g, c->key_perm + i, sizeof g, hipMemcpyDeviceToHost));
+    if ((int64_t)g >= b->n[s]) return PLB_ERR_BOUNDS;
+    uint64_t xyz = 0;  // node_xyz holds each sample's coordinates in its V order
+    PLB_HIP(hipMemcpy(&xyz, c->node_xyz + b->off[s] + g, sizeof xyz, hipMemcpyDeviceToHost));
+    PLB_HIP(hipMemcpy(&out->rho, c->macros + i, sizeof(double), hipMemcpyDeviceToHost));
+    *sample = s;
+    out->tripped = 1;
+    out->step = (int64_t)(key >> 32);
+    out->node = g;
+    out->pid = 0;
+    out->local_id = g;
+    out->x = (int)(xyz & 0xffff);
+    out->y = (int)((xyz >> 16) & 0xffff);
+    out->z = (int)((xyz >> 32) & 0xffff);
+    return PLB_OK;
+}
+
 extern "C" int plb_batch_read_macros(plb_batch* b, int sample, double* rho, double* ux, double* uy,
                                      double* uz)
 {

@@ -64,14 +64,25 @@ def test_convergence_stops_equal_per_step_driver(plb):
         assert np.array_equal(a["mac"][s], b["mac"][s]), s
 
 
-def test_density_guard_in_persistent_launch(plb):
+@pytest.mark.parametrize("kw", [dict(), dict(index_kind=1, layout=1, ordering=2, brick=(64, 4, 1))])
+def test_density_guard_in_persistent_launch(plb, kw):
+    """A density-guard trip in one sample stops both drivers with -6, and
+    plb_batch_last_fault names the sample and reports the node, coordinates,
+    step and rho that the sample's own lattice reports (porelb.c:338-342)."""
     d = dict(np.load(os.path.join(GOLDEN, "fault.npz")))
     fl = [masks(plb)[0], d["flag"]]
+    with plb.Lattice.from_flag(d["flag"], tau=float(d["tau"]), gx=float(d["gx"])) as lat:
+        assert lat.step_rc(0, 200) == plb.ERR_DENSITY
+        want = lat.last_fault()
     for persistent in (False, True):
-        b = plb.Batch(fl, [0.8, float(d["tau"])], [1e-5, float(d["gx"])], precision=0, bb_mode=0)
+        b = plb.Batch(fl, [0.8, float(d["tau"])], [1e-5, float(d["gx"])], precision=0, bb_mode=0, **kw)
         try:
             with pytest.raises(plb.PlbError) as ei:
                 b.run(200, 10, 0.0, persistent=persistent)
             assert ei.value.code == plb.ERR_DENSITY
+            s, f = b.last_fault()
+            assert s == 1
+            for k in ("tripped", "step", "node", "local_id", "x", "y", "z", "rho"):
+                assert f[k] == want[k], (persistent, k)
         finally:
             b.close()
