@@ -37,8 +37,10 @@ STORAGE_BYTES = {0: 8, 1: 4, 2: 2}   # population storage bytes per precision
 # Launch shapes measured on MI355X (plb_autotune candidates and a finer
 # sweep): layout 1 = AoSoA tiles of 64 nodes; index 1 = TILE8 pull table;
 # ordering 2 = pencils of `brick` = (nx, Py, 1) rows visited plane by plane.
+# reserve_gib: device memory the timed lattice allocates first and holds (plb_config.placement_reserve_gib);
+# on a freshly booted MI355X the first ~40 GiB the first process allocates stream about 6 % slower.
 SPHERES1024 = dict(kind="spheres", dims=(1024, 1024, 1024), rmin=6.0, rmax=12.0, porosity=0.20, seed=4,
-                   tau=0.6, gx=5e-7, label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7")
+                   tau=0.6, gx=5e-7, reserve_gib=40, label="1024^3 sphere pack, porosity 0.20, tau 0.6, g_x 5e-7")
 WORKLOADS = {
     "spheres1024-fp32": dict(SPHERES1024, precision=1, bb=1, index=1, layout=1, ordering=2,
                              brick=(1024, 128, 1), block=256, waves=5),
@@ -59,6 +61,9 @@ WORKLOADS = {
 # reference bounce order and the int32 pull table, in its fastest launch shape.
 FP64_PLAIN = dict(precision=0, bb=0, index=0, layout=1, ordering=2, brick=(1024, 64, 1), block=256, waves=5)
 PREC_NAME = {0: "f64", 1: "f32-store/f32-collide", 2: "f16-store/f32-collide"}
+# The paths timed beside the line (plain FP64, FP16 storage) take a fixed
+# window whatever --steps and --warmup are (about 1.4 s and 0.4 s of GPU time).
+SECONDARY_STEPS, SECONDARY_WARMUP = 100, 10
 
 
 def log(*a):
@@ -87,13 +92,13 @@ def make_mask(P, w):
 
 
 def time_path(P, flag, w, precision, bb, steps, warmup, layout=0, ordering=0, block=256, index=0,
-              brick=(16, 4, 4), waves=0, fp16_rounding=1, keep=False):
+              brick=(16, 4, 4), waves=0, fp16_rounding=1, keep=False, reserve_gib=0):
     """Times `steps` steps after `warmup` on a new lattice.  keep=True returns
     the lattice open (result["lattice"]) for the precision gate, so the gate's
     steps run on the allocation that was timed."""
     cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=precision, bb_mode=bb, layout=layout,
                            ordering=ordering, block_size=block, index_kind=index, brick=brick,
-                           waves_per_simd=waves, fp16_rounding=fp16_rounding)
+                           waves_per_simd=waves, fp16_rounding=fp16_rounding, placement_reserve_gib=reserve_gib)
     t0 = time.perf_counter()
     lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
     log(f"lattice built: {lat.N} pore nodes, precision {precision}, {time.perf_counter() - t0:.1f} s")
@@ -500,7 +505,8 @@ def main():
     gate = args.gate_steps > 0 and w["precision"] != 0 and w["bb"] == 1
     main_run = time_path(P, flag, w, w["precision"], w["bb"], args.steps, args.warmup, shape["layout"],
                          shape["ordering"], shape["block"], shape["index"], w.get("brick", (16, 4, 4)),
-                         shape["waves"], w.get("fp16_rounding", 1), keep=gate)
+                         shape["waves"], w.get("fp16_rounding", 1), keep=gate,
+                         reserve_gib=w.get("reserve_gib", 0))
     kept = {w["precision"]: main_run.pop("lattice")} if gate else {}
     n, steps = main_run["nodes"], args.steps
     mlups = n * steps / main_run["wall_s"] / 1e6
@@ -530,12 +536,12 @@ def main():
     if not args.skip_fp64 and w["precision"] != 0:
         fp = FP64_PLAIN
         brick = (w["dims"][2],) + fp["brick"][1:]
-        fp64 = time_path(P, flag, w, 0, 0, max(20, steps // 2), max(5, args.warmup // 2), fp["layout"],
+        fp64 = time_path(P, flag, w, 0, 0, SECONDARY_STEPS, SECONDARY_WARMUP, fp["layout"],
                          fp["ordering"], fp["block"], fp["index"], brick, fp["waves"])
-        fp64_mlups = fp64["nodes"] * max(20, steps // 2) / fp64["wall_s"] / 1e6
+        fp64_mlups = fp64["nodes"] * SECONDARY_STEPS / fp64["wall_s"] / 1e6
         secondary = {"fp64_plain_mlups": round(fp64_mlups, 1),
                      "fp64_plain_roofline_frac": round((2 * 19 * 8 + 19 * 4) * fp64["nodes"] /
-                                                       (fp64["event_ms"] / max(20, steps // 2) * 1e-3) /
+                                                       (fp64["event_ms"] / SECONDARY_STEPS * 1e-3) /
                                                        1e9 / HBM_PEAK_GBS, 4),
                      "speedup_vs_fp64_plain": round(mlups / fp64_mlups, 3),
                      "fp64_plain_shape": "int32 table, AoSoA, pencils of 64 rows, 256 threads, "
@@ -544,8 +550,8 @@ def main():
     # FP16 storage (stochastic rounding) in its own fastest shape, beside the line.
     if not args.skip_fp16 and w["precision"] == 1:
         w16 = WORKLOADS["spheres1024-fp16"]
-        st16 = max(20, steps // 2)
-        r16 = time_path(P, flag, w, 2, 1, st16, max(5, args.warmup // 2), w16["layout"], w16["ordering"],
+        st16 = SECONDARY_STEPS
+        r16 = time_path(P, flag, w, 2, 1, st16, SECONDARY_WARMUP, w16["layout"], w16["ordering"],
                         w16["block"], w16["index"], w16["brick"], 0, w16["fp16_rounding"], keep=gate)
         if gate:
             kept[2] = r16.pop("lattice")
@@ -595,7 +601,7 @@ def main():
                    "layout": "AoSoA64" if shape["layout"] else "SoA",
                    "node_order": ["reference", "bricks", "pencils"][shape["ordering"]],
                    "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"],
-                   "waves_per_simd": shape["waves"]},
+                   "waves_per_simd": shape["waves"], "placement_reserve_gib": w.get("reserve_gib", 0)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": (f"{tname}, rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, measured at commit "

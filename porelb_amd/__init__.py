@@ -54,7 +54,7 @@ class Config(C.Structure):
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
                 ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int),
                 ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int), ("waves_per_simd", C.c_int),
-                ("fp16_rounding", C.c_int)]
+                ("fp16_rounding", C.c_int), ("placement_reserve_gib", C.c_int)]
 
 
 class Reduction(C.Structure):

@@ -99,6 +99,7 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->waves_per_simd < 0 || cfg->waves_per_simd > 8) return PLB_ERR_INVALID;
     if (cfg->fp16_rounding != PLB_ROUND_NEAREST && cfg->fp16_rounding != PLB_ROUND_STOCHASTIC)
         return PLB_ERR_INVALID;
+    if (cfg->placement_reserve_gib < 0 || cfg->placement_reserve_gib > 160) return PLB_ERR_INVALID;
     return PLB_OK;
 }
 
@@ -134,6 +135,8 @@ int plb::ctx_begin(plb_ctx* c, const plb_config* cfg, int nx, int ny, int nz, in
     const int rc = noalloc_setup(c->device);  // the no-alloc read-back's staging array
     if (rc) return rc;
     PLB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (cfg->placement_reserve_gib > 0)  // before every buffer of the context
+        PLB_HIP(hipMalloc(&c->reserve, (size_t)cfg->placement_reserve_gib << 30));
     PLB_HIP(hipEventCreate(&c->ev0));
     PLB_HIP(hipEventCreate(&c->ev1));
     return PLB_OK;
@@ -165,7 +168,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipStreamSynchronize(c->stream);
     }
     void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
-                    c->inv_perm, c->key_perm, c->node_xyz, c->macros, c->fault, c->red};
+                    c->inv_perm, c->key_perm, c->node_xyz, c->macros, c->fault, c->red, c->reserve};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -66,13 +66,18 @@ __global__ void k_batch_inv(uint32_t* inv, const uint32_t* sample_inv, int64_t n
     inv[off + g] = (uint32_t)(off + (sample_inv ? (int64_t)sample_inv[g] : g));
 }
 
-// key[off + (sample order of V-order node g)] = g: each node's V-order id
-// within its sample (keys the FP16 stochastic-rounding stream).
-__global__ void k_batch_key(uint32_t* key, const uint32_t* sample_inv, int64_t n, int64_t off)
+// For the node at internal index k = off + (sample order of V-order node g):
+// key[k] = g, its V-order id within its sample (keys the FP16
+// stochastic-rounding stream), and perm[k] = off + g, its V-order id in the
+// concatenation (the density guard reports the lowest of these, as the
+// reference reports the first node of its V-order loop).
+__global__ void k_batch_key(uint32_t* key, uint32_t* perm, const uint32_t* sample_inv, int64_t n, int64_t off)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    key[off + (sample_inv ? (int64_t)sample_inv[g] : g)] = (uint32_t)g;
+    const int64_t k = off + (sample_inv ? (int64_t)sample_inv[g] : g);
+    key[k] = (uint32_t)g;
+    perm[k] = (uint32_t)(off + g);
 }
 
 static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* smp)
@@ -121,6 +126,8 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
         if (e == hipSuccess) e = hipMalloc(&c->inv_perm, sizeof(uint32_t) * total);
         if (e == hipSuccess) e = hipMalloc(&c->key_perm, sizeof(uint32_t) * total);
         if (e == hipSuccess) e = hipMemsetAsync(c->key_perm, 0, sizeof(uint32_t) * total, c->stream);
+        if (e == hipSuccess) e = hipMalloc(&c->perm, sizeof(uint32_t) * total);
+        if (e == hipSuccess) e = hipMemsetAsync(c->perm, 0, sizeof(uint32_t) * total, c->stream);
         if (e != hipSuccess) rc = e == hipErrorOutOfMemory ? PLB_ERR_NOMEM : PLB_ERR_HIP;
     }
     for (int s = 0; s < b->S && !rc; s++) {
@@ -133,7 +140,7 @@ static int batch_build(plb_batch* b, const plb_config* cfg, const plb_sample* sm
             hipLaunchKernelGGL(k_batch_inv, dim3((unsigned)((b->n[s] + 255) / 256)), dim3(256), 0, c->stream,
                                c->inv_perm, invs[s], b->n[s], b->off[s]);
             hipLaunchKernelGGL(k_batch_key, dim3((unsigned)((b->n[s] + 255) / 256)), dim3(256), 0, c->stream,
-                               c->key_perm, invs[s], b->n[s], b->off[s]);
+                               c->key_perm, c->perm, invs[s], b->n[s], b->off[s]);
             if (hipGetLastError() != hipSuccess) rc = PLB_ERR_HIP;
         }
     }
@@ -452,15 +459,16 @@ extern "C" int plb_batch_last_fault(plb_batch* b, int* sample, plb_fault* out)
     unsigned long long key = NO_FAULT;
     PLB_HIP(hipMemcpy(&key, c->fault, sizeof key, hipMemcpyDeviceToHost));
     if (key == NO_FAULT) return PLB_OK;
-    // the key holds the internal (concatenated) index of the node
-    const int64_t i = (int64_t)(key & 0xffffffffull);
+    // the key holds the node's V-order index in the concatenation
+    const int64_t gc = (int64_t)(key & 0xffffffffull);
     int s = -1;
     for (int q = 0; q < b->S; q++)
-        if (i >= b->off[q] && i < b->off[q] + b->n[q]) s = q;
+        if (gc >= b->off[q] && gc < b->off[q] + b->n[q]) s = q;
     if (s < 0) return PLB_ERR_BOUNDS;
-    uint32_t g = 0;  // V-order id within the sample
-    PLB_HIP(hipMemcpy(&g, c->key_perm + i, sizeof g, hipMemcpyDeviceToHost));
-    if ((int64_t)g >= b->n[s]) return PLB_ERR_BOUNDS;
+    const int64_t g = gc - b->off[s];  // V-order id within the sample
+    uint32_t i = 0;                    // its internal index (the macros' position)
+    PLB_HIP(hipMemcpy(&i, c->inv_perm + gc, sizeof i, hipMemcpyDeviceToHost));
+    if ((int64_t)i >= c->Npad) return PLB_ERR_BOUNDS;
     uint64_t xyz = 0;  // node_xyz holds each sample's coordinates in its V order
     PLB_HIP(hipMemcpy(&xyz, c->node_xyz + b->off[s] + g, sizeof xyz, hipMemcpyDeviceToHost));
     PLB_HIP(hipMemcpy(&out->rho, c->macros + i, sizeof(double), hipMemcpyDeviceToHost));

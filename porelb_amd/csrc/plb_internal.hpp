@@ -131,6 +131,7 @@ struct plb_ctx {
     uint32_t* inv_perm = nullptr;
     uint64_t* node_xyz = nullptr;  // internal order
     double* macros = nullptr;   // [4][Npad]
+    void* reserve = nullptr;    // placement reserve (cfg.placement_reserve_gib)
     unsigned long long* fault = nullptr;
     double* red = nullptr;      // reduction partials
     int red_blocks = 0;

@@ -22,6 +22,7 @@
 //   S_j = (1 - omega/2)*3*(e_jx - u)*F,
 // which is the reference update minus w_j*rho0 on both sides.
 #include <algorithm>
+#include <type_traits>
 
 #include "plb_internal.hpp"
 
@@ -317,6 +318,7 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x)
     x ^= x >> 16;
     return x;
 }
+__device__ __forceinline__ void st_dev_sr(float* p, int64_t k, float v, uint32_t) { st_dev(p, k, v); }
 __device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, uint32_t r)
 {
     const plb_half2 h = __builtin_amdgcn_cvt_sr_f16_f32(plb_half2{0, 0}, v, r, false);
@@ -333,7 +335,7 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     float ds[Q];
 #pragma unroll
     for (int j = 0; j < Q; j++) ds[j] = to_float(raw[j]);
-    if (a.has_const) {  // constant entries exist only under the reference bounce order
+    if (__builtin_amdgcn_readfirstlane(a.has_const)) {  // constant entries: reference bounce order only
 #pragma unroll
         for (int j = 1; j < Q; j++)
             if (t[j] == TAB_CONST) ds[j] = 0.f;
@@ -380,29 +382,36 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     const float f0s = scaled ? fu * sc : fu, fps = scaled ? (fu + c2) * sc : fu + c2,
                 fms = scaled ? (fu - c2) * sc : fu - c2;
     S* __restrict__ onode = fout + fpos<LAYOUT>(0, i, a.Npad);
-    uint32_t rnd = 0;
-    if constexpr (scaled)
-        if (a.stochastic) rnd = mix32(key ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
+    // The rounding mode is uniform: one loop per mode under a scalar branch.
+    auto relax_and_store = [&](auto stochastic) {
+        constexpr bool SR = decltype(stochastic)::value;
+        uint32_t rnd = 0;
+        if constexpr (SR) rnd = mix32(key ^ mix32((uint32_t)a.it * 0x9e3779b9u + 0x632be5abu));
 #pragma unroll
-    for (int j = 0; j < Q; j++) {
-        float eu = 0.f;
-        if (EX[j]) eu = EX[j] > 0 ? u : -u;
-        if (EY[j]) eu += EY[j] > 0 ? v : -v;
-        if (EZ[j]) eu += EZ[j] > 0 ? w : -w;
-        const float P = fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
-        const float wj = (float)TP[j];
-        const float dfeq = wj * fmaf(rho, P, drho);
-        const float feq = fmaf(wj, rho0, dfeq);
-        const float frc = EX[j] > 0 ? fps : (EX[j] < 0 ? fms : f0s);
-        const float o = fmaf(c1, ds[j], fmaf(omega_s, dfeq, frc * feq));
-        if constexpr (scaled) {
-            if (a.stochastic)
+        for (int j = 0; j < Q; j++) {
+            float eu = 0.f;
+            if (EX[j]) eu = EX[j] > 0 ? u : -u;
+            if (EY[j]) eu += EY[j] > 0 ? v : -v;
+            if (EZ[j]) eu += EZ[j] > 0 ? w : -w;
+            const float P = fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
+            const float wj = (float)TP[j];
+            const float dfeq = wj * fmaf(rho, P, drho);
+            const float feq = fmaf(wj, rho0, dfeq);
+            const float frc = EX[j] > 0 ? fps : (EX[j] < 0 ? fms : f0s);
+            const float o = fmaf(c1, ds[j], fmaf(omega_s, dfeq, frc * feq));
+            if constexpr (SR)
                 st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, rnd + (uint32_t)j * 0x9e3779b9u);
             else
                 st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
-        } else {
-            st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
         }
+    };
+    if constexpr (scaled) {
+        if (__builtin_amdgcn_readfirstlane(a.stochastic))
+            relax_and_store(std::true_type{});
+        else
+            relax_and_store(std::false_type{});
+    } else {
+        relax_and_store(std::false_type{});
     }
 }
 
