@@ -436,6 +436,44 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
     print(json.dumps(out), flush=True)
 
 
+def study_placement(P, reserve_gib, steps=200, warmup=20):
+    """Which allocations of the FP32 1024^3 lattice stream slower: after the
+    benchmark's mask and copy-bandwidth run, hold `reserve_gib` GiB, then time
+    lattice A, lattice B while A is alive, and lattice C after A is freed.
+    Run it as the first and later processes on a freshly booted box."""
+    import ctypes
+
+    w = WORKLOADS["spheres1024-fp32"]
+    flag = make_mask(P, w)
+    P.copy_bandwidth(4 << 30, 5)
+    hip = ctypes.CDLL("libamdhip64.so")
+    held = ctypes.c_void_p()
+    nbytes = ctypes.c_size_t(int(reserve_gib) << 30)
+    if reserve_gib > 0:
+        if hip.hipMalloc(ctypes.byref(held), nbytes) != 0 or hip.hipMemset(held, 0, nbytes) != 0:
+            raise RuntimeError("hipMalloc of the held block failed")
+        hip.hipDeviceSynchronize()
+
+    def timed():
+        cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=1, bb_mode=1, layout=1, ordering=2,
+                               block_size=256, index_kind=1, brick=w["brick"], waves_per_simd=5)
+        lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
+        lat.step(0, warmup)
+        ms = lat.step_timed(warmup, steps)
+        return lat, round(lat.N * steps / (ms * 1e-3) / 1e6, 1)
+
+    out = {"study": "placement", "held_gib": reserve_gib, "workload": w["label"], "steps": steps}
+    a, out["A_mlups"] = timed()
+    b, out["B_A_alive_mlups"] = timed()
+    a.close()
+    c, out["C_A_freed_mlups"] = timed()
+    b.close()
+    c.close()
+    if reserve_gib > 0:
+        hip.hipFree(held)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -450,12 +488,14 @@ def main():
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
     ap.add_argument("--waves", type=int, default=None, help="step-kernel waves per SIMD (0 = no cap)")
-    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess", "outputs"], default=None,
+    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess", "outputs", "placement"],
+                    default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--check", type=int, default=200)
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--samples", type=int, default=64)
+    ap.add_argument("--reserve-gib", type=int, default=0, help="placement study: GiB held before the lattices")
     ap.add_argument("--gate-steps", type=int, default=5000,
                     help="steps of the FP64 vs reduced-precision comparison in the line (0 = skip)")
     args = ap.parse_args()
@@ -472,6 +512,9 @@ def main():
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6,
                     waves=args.waves if args.waves is not None else 5)
+        return
+    if args.study == "placement":
+        study_placement(P, args.reserve_gib)
         return
     if args.study == "outputs":
         study_outputs(P)

@@ -17,6 +17,7 @@
  *   PLB_TAU (0.8)  PLB_GX (1e-3)  PLB_RHO0 (1)  PLB_SAVE (1000)
  *   PLB_PRECISION fp64|fp32|fp16 (fp64)  PLB_BB refcompat|halfway (refcompat)
  *   PLB_FP16_ROUND nearest|stochastic (stochastic): rounding of FP16 stores
+ *   PLB_RESERVE_GIB (0): device memory held below the lattice (placement_reserve_gib)
  *   PLB_LAYOUT soa|aosoa (aosoa)  PLB_ORDER reference|brick|pencil (pencil)
  *   PLB_BRICK bx,by,bz (pencil: nx,64,1)  PLB_INDEX i32|tile8|bitmap (tile8)
  *   PLB_BLOCK (128 below 4 M pore nodes, else 256)  PLB_GRAPH (0: steps per graph replay)
@@ -221,6 +222,7 @@ int main(int argc, char *argv[])
     cfg.waves_per_simd =
         env_int("PLB_WAVES", cfg.precision != PLB_FP16 && v.num_node >= (1 << 22) ? 5 : 0);
     cfg.fp16_rounding = env_choice("PLB_FP16_ROUND", "nearest", "stochastic", NULL, PLB_ROUND_STOCHASTIC);
+    cfg.placement_reserve_gib = env_int("PLB_RESERVE_GIB", 0);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
     const int print = env_int("PLB_PRINT", 1);
