@@ -245,7 +245,8 @@ int plb_last_fault(plb_ctx *c, plb_fault *out);
 
 /* Bytes per node update the step kernel moves as algorithmic traffic:
  * populations read + written (2*19*element size) plus the pull-table bytes
- * of the context's encoding per node. */
+ * of the context's encoding per node (and, for FP16 storage with stochastic
+ * rounding in a node order other than V order, the 4-byte rounding key). */
 double plb_bytes_per_update(const plb_ctx *c);
 
 const char *plb_strerror(int code);

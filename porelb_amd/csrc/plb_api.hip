@@ -376,7 +376,10 @@ extern "C" void plb_host_free(void* p)
 extern "C" double plb_bytes_per_update(const plb_ctx* c)
 {
     if (!c) return 0.0;
-    return 2.0 * Q * (double)c->elem + c->index_bytes;
+    // FP16 storage with stochastic rounding also loads each node's 4-byte V-order key
+    const bool key = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC &&
+                     (c->perm || c->key_perm);
+    return 2.0 * Q * (double)c->elem + c->index_bytes + (key ? 4.0 : 0.0);
 }
 
 static const int64_t CHUNK_NODES = 1 << 20;
