@@ -388,27 +388,69 @@ def batch_samples(P, n_samples=64, seed=100):
     return flags
 
 
-def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
+def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0, tune_steps=20):
     """Config 5: n samples, edge L ~ U{64..192}, porosity ~ U[0.15, 0.45], half
     sphere packs (r U[6,12]) and half smoothed noise (sigma 2), FP32 storage,
-    TILE8, tau 1.0, g_x 1e-6.  One batched launch per step over all active
-    samples (per-sample convergence stop) vs the same samples run one after
-    another (one sample per launch, same kernels and convergence rule); at
-    most `waves` step-kernel waves per SIMD in both."""
+    half-way bounce-back, tau 1.0, g_x 1e-6.  One batched launch per step over
+    all active samples (per-sample convergence stop) vs the same samples run
+    one after another (one sample per launch, same kernels and convergence
+    rule); at most `waves` step-kernel waves per SIMD in both (TILE8, pencils
+    of 32 rows, 256 threads).  Then the same per-step batch in the shape
+    plb_batch_autotune picks for the whole batch, and as one batch per size
+    class (L <= 128, L > 128), each in the shape tuned on that class."""
     flags = batch_samples(P, n_samples, seed)
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
     kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256,
               waves_per_simd=waves)
     modes = {}
+
+    def run_batch(idx, cfg, persistent=False):
+        b = P.Batch([flags[k] for k in idx], [taus[k] for k in idx], [gxs[k] for k in idx], cfg=cfg)
+        try:
+            steps, upd, ms = b.run(max_steps, check, tol, persistent=persistent)
+            red = b.reduce()
+            nodes = list(b.nodes)
+        finally:
+            b.close()
+        return dict(steps=[int(x) for x in steps], upd=int(upd), ms=ms, K=[r["K"] for r in red], nodes=nodes)
+
+    everyone = list(range(n_samples))
     for persistent in (False, True):
-        b = P.Batch(flags, taus, gxs, **kw)
-        steps, upd, ms = b.run(max_steps, check, tol, persistent=persistent)
-        red = b.reduce()
-        nodes = list(b.nodes)
-        b.close()
-        modes["persistent" if persistent else "per_step"] = dict(steps=[int(x) for x in steps], upd=int(upd), ms=ms,
-                                                                K=[r["K"] for r in red])
-        log(f"batch {'persistent' if persistent else 'per-step'}: {ms:.1f} ms, {upd / (ms * 1e-3) / 1e6:.0f} MLUPS")
+        m = run_batch(everyone, P.default_config(**kw), persistent)
+        nodes = m.pop("nodes")
+        modes["persistent" if persistent else "per_step"] = m
+        log(f"batch {'persistent' if persistent else 'per-step'}: {m['ms']:.1f} ms, "
+            f"{m['upd'] / (m['ms'] * 1e-3) / 1e6:.0f} MLUPS")
+
+    def shape(cfg):
+        return {"index_kind": cfg.index_kind, "layout": cfg.layout, "ordering": cfg.ordering,
+                "pencil_rows": cfg.brick[1] if cfg.ordering == 2 else None, "block": cfg.block_size,
+                "waves_per_simd": cfg.waves_per_simd}
+
+    base = P.default_config(**kw)
+    t0 = time.perf_counter()
+    tuned_cfg, _ = P.batch_autotune(flags, taus, gxs, tune_steps, cfg=base)
+    tune_s = time.perf_counter() - t0
+    tuned = run_batch(everyone, tuned_cfg)
+    tuned.pop("nodes")
+    log(f"batch tuned {shape(tuned_cfg)}: {tuned['ms']:.1f} ms ({tune_s:.1f} s of tuning)")
+    classes = {"L<=128": [k for k in everyone if max(flags[k].shape) <= 128],
+               "L>128": [k for k in everyone if max(flags[k].shape) > 128]}
+    per_class = {"updates": 0, "ms": 0.0, "classes": {}}
+    class_steps, class_K = [0] * n_samples, [0.0] * n_samples
+    for name, idx in classes.items():
+        if not idx:
+            continue
+        ccfg, _ = P.batch_autotune([flags[k] for k in idx], [taus[k] for k in idx], [gxs[k] for k in idx],
+                                   tune_steps, cfg=base)
+        m = run_batch(idx, ccfg)
+        for j, k in enumerate(idx):
+            class_steps[k], class_K[k] = m["steps"][j], m["K"][j]
+        per_class["updates"] += m["upd"]
+        per_class["ms"] += m["ms"]
+        per_class["classes"][name] = dict(samples=len(idx), shape=shape(ccfg),
+                                          mlups=round(m["upd"] / (m["ms"] * 1e-3) / 1e6, 1))
+        log(f"batch class {name} ({len(idx)} samples) {shape(ccfg)}: {m['ms']:.1f} ms")
     seq_upd, seq_ms, seq_steps = 0, 0.0, []
     for k, f in enumerate(flags):
         one = P.Batch([f], [taus[k]], [gxs[k]], **kw)
@@ -433,6 +475,15 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0):
            "K_range": [float(min(per["K"])), float(max(per["K"]))]}
     out["speedup_per_step_vs_sequential"] = round(out["per_step"]["mlups"] / out["sequential"]["mlups"], 3)
     out["speedup_persistent_vs_sequential"] = round(out["persistent"]["mlups"] / out["sequential"]["mlups"], 3)
+    # Node values are shape-independent; the FP64 sums behind K and the stop
+    # test follow the node order, so K may differ in its last bits.
+    def kdiff(K):
+        return float(max(abs(a - b) / abs(b) for a, b in zip(K, per["K"]) if b != 0.0))
+
+    out["tuned"] = dict(rate(tuned["upd"], tuned["ms"]), shape=shape(tuned_cfg), tuning_s=round(tune_s, 1),
+                        same_steps=tuned["steps"] == per["steps"], K_max_rel_diff=kdiff(tuned["K"]))
+    out["tuned_per_class"] = dict(rate(per_class["updates"], per_class["ms"]), classes=per_class["classes"],
+                                  same_steps=class_steps == per["steps"], K_max_rel_diff=kdiff(class_K))
     print(json.dumps(out), flush=True)
 
 

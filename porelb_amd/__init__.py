@@ -132,6 +132,7 @@ def _lib():
             "plb_mask_bcc": ([p, i32, d], i32),
             "plb_copy_bandwidth": ([i32, i64, i32, p], i32),
             "plb_autotune": ([p, p, i32, i32, i32, i32, i32, C.c_char_p, p, p, i32, p], i32),
+            "plb_batch_autotune": ([p, p, i32, i32, p, p, i32, p], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -381,13 +382,40 @@ def autotune(flag, num_p=1, steps=20, cache_path=None, cfg=None, **kw):
     _check(_lib().plb_autotune(C.byref(cfg), _ptr(flag), nx, ny, nz, num_p, steps,
                                cache_path.encode() if cache_path else None, C.byref(best), res, 64,
                                C.byref(n)), "plb_autotune")
+    return best, _tune_rows(res[:n.value])
+
+
+def _tune_rows(res):
     out = []
-    for r in res[:n.value]:
+    for r in res:
         c = r.cfg
         out.append(dict(index_kind=c.index_kind, layout=c.layout, ordering=c.ordering, brick=tuple(c.brick),
                         block=c.block_size, npt=c.nodes_per_thread, waves=c.waves_per_simd,
                         mlups=r.mlups, ms=r.ms))
-    return best, out
+    return out
+
+
+def _sample_array(flags, taus, gxs):
+    flags = [np.ascontiguousarray(f, dtype=np.uint8) for f in flags]
+    arr = (Sample * len(flags))()
+    for s, (f, tau, gx) in enumerate(zip(flags, taus, gxs)):
+        nz, ny, nx = f.shape
+        arr[s] = Sample(f.ctypes.data, nx, ny, nz, tau, gx)
+    return flags, arr
+
+
+def batch_autotune(flags, taus, gxs, steps=20, cfg=None, **kw):
+    """Fastest (index encoding, layout, node order, block size, waves per
+    SIMD) for a batch of these samples, by plb_batch_autotune's three-stage
+    search; returns (Config, results)."""
+    cfg = cfg or default_config(**kw)
+    keep, arr = _sample_array(flags, taus, gxs)
+    best = Config()
+    res = (TuneResult * 64)()
+    n = C.c_int(0)
+    _check(_lib().plb_batch_autotune(C.byref(cfg), arr, len(keep), steps, C.byref(best), res, 64, C.byref(n)),
+           "plb_batch_autotune")
+    return best, _tune_rows(res[:n.value])
 
 
 # ------------------------------------------------------------------ batch
@@ -396,11 +424,7 @@ class Batch:
 
     def __init__(self, flags, taus, gxs, cfg=None, **kw):
         cfg = cfg or default_config(**kw)
-        self._flags = [np.ascontiguousarray(f, dtype=np.uint8) for f in flags]
-        arr = (Sample * len(self._flags))()
-        for s, (f, tau, gx) in enumerate(zip(self._flags, taus, gxs)):
-            nz, ny, nx = f.shape
-            arr[s] = Sample(f.ctypes.data, nx, ny, nz, tau, gx)
+        self._flags, arr = _sample_array(flags, taus, gxs)
         h = C.c_void_p()
         _check(_lib().plb_batch_create(C.byref(h), C.byref(cfg), arr, len(self._flags)),
                "plb_batch_create")

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "plb_internal.hpp"
+#include "plb_tune.h"
 
 using namespace plb;
 
@@ -534,5 +535,101 @@ extern "C" int plb_batch_reduce(plb_batch* b, plb_reduction* out)
         const double nu = 1.0 / 3.0 * (b->tau[s] - 0.5);
         o.K = b->gx[s] != 0.0 ? nu * o.q / b->gx[s] : 0.0;
     }
+    return PLB_OK;
+}
+
+// Launch-shape search for a batch: every candidate gives the same per-sample
+// results (the shapes change the node order inside each sample, the index
+// encoding and the launch, never a source or a rounding), so only the time of
+// `steps` fixed steps (tol 0: no sample stops) is compared.  Stage 1 builds
+// one batch per index encoding x layout x node order at the base block size;
+// stage 2 one batch per block size for the winner; stage 3 times the wave
+// caps on the stage-2 winner's batch, since they change only the launch.
+extern "C" int plb_batch_autotune(const plb_config* base, const plb_sample* samples, int n_samples, int steps,
+                                  plb_config* best, plb_tune_result* results, int max_results, int* n_results)
+{
+    if (!base || !samples || n_samples < 1 || !best || steps < 1 || max_results < 0) return PLB_ERR_INVALID;
+    Range range("plb_batch_autotune");
+    int rc = check_cfg(base);
+    if (rc) return rc;
+    if (n_results) *n_results = 0;
+    *best = *base;
+    best->nodes_per_thread = 1;
+    best->graph_steps = 0;
+    double best_mlups = -1.0;
+    int nres = 0;
+    auto record = [&](const plb_config& cfg, int64_t upd, double ms) {
+        const double m = ms > 0.0 ? (double)upd / (ms * 1e-3) / 1e6 : 0.0;
+        if (results && nres < max_results) results[nres] = {cfg, m, ms};
+        nres++;
+        if (m > best_mlups) {
+            best_mlups = m;
+            *best = cfg;
+        }
+    };
+    // One warm-up run, then the timed run; returns the open batch in *keep
+    // when asked (stage 3 reuses it).
+    auto time_batch = [&](const plb_config& cfg, plb_batch** keep) -> int {
+        plb_batch* b = nullptr;
+        int r = plb_batch_create(&b, &cfg, samples, n_samples);
+        if (r) return r;
+        int64_t upd = 0;
+        double ms = 0.0;
+        r = plb_batch_run(b, std::max(2, steps / 4), steps, 0.0, nullptr, &upd, &ms);
+        if (!r) r = plb_batch_run(b, steps, steps, 0.0, nullptr, &upd, &ms);
+        if (!r) record(cfg, upd, ms);
+        if (r || !keep)
+            plb_batch_destroy(b);
+        else
+            *keep = b;
+        return r;
+    };
+    int max_nx = 1;
+    for (int s = 0; s < n_samples; s++) max_nx = std::max(max_nx, samples[s].nx);
+    // node orders: V order; pencils of 16, 32 and 64 full rows (a pencil
+    // takes each sample's own x extent)
+    const int rows[] = {0, 16, 32, 64};
+    for (int idx : {PLB_INDEX_I32, PLB_INDEX_TILE8})
+        for (int layout = 0; layout < 2; layout++)
+            for (int py : rows) {
+                plb_config cfg = *best;
+                cfg.index_kind = idx;
+                cfg.layout = layout;
+                cfg.block_size = base->block_size;
+                cfg.waves_per_simd = 0;
+                cfg.ordering = py ? PLB_ORDER_PENCIL : PLB_ORDER_REFERENCE;
+                if (py) {
+                    cfg.brick[0] = max_nx;
+                    cfg.brick[1] = py;
+                    cfg.brick[2] = 1;
+                }
+                rc = time_batch(cfg, nullptr);
+                if (rc) return rc;
+            }
+    const plb_config stage1 = *best;
+    plb_batch* keep = nullptr;
+    for (int bs : {128, 256, 512}) {
+        if (bs == stage1.block_size) continue;
+        plb_config cfg = stage1;
+        cfg.block_size = bs;
+        rc = time_batch(cfg, nullptr);
+        if (rc) return rc;
+    }
+    const plb_config stage2 = *best;
+    rc = time_batch(stage2, &keep);  // the winner's batch, timed again beside its wave caps
+    if (rc) return rc;
+    for (int waves : {6, 5, 4}) {
+        plb_config cfg = stage2;
+        cfg.waves_per_simd = waves;
+        keep->c->cfg.waves_per_simd = waves;
+        int64_t upd = 0;
+        double ms = 0.0;
+        rc = plb_batch_run(keep, steps, steps, 0.0, nullptr, &upd, &ms);
+        if (rc) break;
+        record(cfg, upd, ms);
+    }
+    plb_batch_destroy(keep);
+    if (rc) return rc;
+    if (n_results) *n_results = std::min(nres, max_results);
     return PLB_OK;
 }

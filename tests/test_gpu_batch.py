@@ -52,3 +52,31 @@ def test_batch_convergence_stops_per_sample(plb):
             assert lat.reduce()["sum_ux"] == red[s]["sum_ux"]
             # the sample's macros are those of its own last step, in V order
             assert np.array_equal(b.read_macros(s), lat.read_macros()), s
+
+
+def test_batch_autotune_shape_keeps_results(plb):
+    """plb_batch_autotune: 16 stage-1 batches (2 index kinds x 2 layouts x 4
+    node orders), 2 other block sizes, the winner again and 3 wave caps; the
+    shape it picks gives every sample the macros of the default shape."""
+    fl = masks(plb)
+    taus, gxs = [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6]
+    base = plb.default_config(precision=1, bb_mode=plb.BB_HALFWAY, block_size=256)
+    best, res = plb.batch_autotune(fl, taus, gxs, steps=8, cfg=base)
+    assert len(res) == 16 + 2 + 1 + 3 and all(r["mlups"] > 0 for r in res)
+    assert {(r["index_kind"], r["layout"]) for r in res[:16]} == {(0, 0), (0, 1), (1, 0), (1, 1)}
+    assert {r["block"] for r in res[16:18]} == {128, 512}
+    assert [r["waves"] for r in res[19:]] == [6, 5, 4]
+    assert best.nodes_per_thread == 1 and best.precision == 1 and best.bb_mode == plb.BB_HALFWAY
+    top = max(res, key=lambda r: r["mlups"])
+    assert (top["index_kind"], top["layout"], top["ordering"], top["block"], top["waves"]) == \
+        (best.index_kind, best.layout, best.ordering, best.block_size, best.waves_per_simd)
+    out = []
+    for cfg in (base, best):
+        b = plb.Batch(fl, taus, gxs, cfg=cfg)
+        try:
+            b.run(120, 40, 0.0)
+            out.append([b.read_macros(s) for s in range(len(fl))])
+        finally:
+            b.close()
+    for s in range(len(fl)):
+        assert np.array_equal(out[0][s], out[1][s]), s
