@@ -487,6 +487,26 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0, tune_ste
     print(json.dumps(out), flush=True)
 
 
+def study_batch_steps(P, steps, waves=4, seed=100):
+    """The 64-sample batch for a fixed number of steps (tol 0: no sample
+    stops), once per step and as one persistent launch, in one shape (FP32,
+    TILE8, AoSoA, pencils of 32 rows, 256 threads, at most `waves` waves per
+    SIMD): a short run for counter passes that compare the two drivers."""
+    flags = batch_samples(P, 64, seed)
+    cfg = P.default_config(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1),
+                           block_size=256, waves_per_simd=waves)
+    b = P.Batch(flags, [1.0] * 64, [1e-6] * 64, cfg=cfg)
+    try:
+        _, u1, m1 = b.run(steps, steps, 0.0)
+        _, u2, m2 = b.run(steps, steps, 0.0, persistent=True)
+    finally:
+        b.close()
+    print(json.dumps({"study": "batch-steps", "pore_nodes_total": int(sum(b.nodes)), "steps": steps,
+                      "waves_per_simd": waves,
+                      "per_step": {"ms": round(m1, 1), "mlups": round(u1 / (m1 * 1e-3) / 1e6, 1)},
+                      "persistent": {"ms": round(m2, 1), "mlups": round(u2 / (m2 * 1e-3) / 1e6, 1)}}), flush=True)
+
+
 def study_placement(P, reserve_gib, steps=200, warmup=20):
     """Which allocations of the FP32 1024^3 lattice stream slower: after the
     benchmark's mask and copy-bandwidth run, hold `reserve_gib` GiB, then time
@@ -539,7 +559,8 @@ def main():
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--index", type=int, default=None)
     ap.add_argument("--waves", type=int, default=None, help="step-kernel waves per SIMD (0 = no cap)")
-    ap.add_argument("--study", choices=["precision", "batch", "config4", "preprocess", "outputs", "placement"],
+    ap.add_argument("--study", choices=["precision", "batch", "batch-steps", "config4", "preprocess", "outputs",
+                                        "placement"],
                     default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
@@ -563,6 +584,9 @@ def main():
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6,
                     waves=args.waves if args.waves is not None else 5)
+        return
+    if args.study == "batch-steps":
+        study_batch_steps(P, args.max_steps if args.max_steps != 100000 else 200)
         return
     if args.study == "placement":
         study_placement(P, args.reserve_gib)
@@ -672,6 +696,11 @@ def main():
                 "K": runs[prec]["K"], "K_rel_err": err["K_rel_err"], "pass": err["K_rel_err"] <= 1e-4,
                 "velocity_rel_l2_err": err["velocity_rel_l2_err"], "mass_rel_drift": runs[prec]["mass_drift"]}
         precision["wall_s"] = round(time.perf_counter() - t0, 1)
+        # The gate's FP64 run is the fastest FP64 form (TILE8 index, half-way
+        # bounce-back), timed by device events over its steps: the line also
+        # states its speedup over that one, beside the plain FP64 path's.
+        secondary["fp64_tile8_halfway_mlups"] = round(runs[0]["mlups"], 1)
+        secondary["speedup_vs_fp64_tile8_halfway"] = round(mlups / runs[0]["mlups"], 3)
         del runs
     for lat in kept.values():
         lat.close()
