@@ -19,6 +19,7 @@ method under mpirun on a bounded sample of the same kind of domain.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import subprocess
@@ -609,6 +610,8 @@ def main():
             log("cpu baseline failed:", repr(e))
     t0 = time.perf_counter()
     flag = make_mask(P, w)
+    # the mask's flag.bin bytes (u8 [nz][ny][nx], x fastest) identify the input
+    mask_sha256 = hashlib.sha256(memoryview(flag).cast("B")).hexdigest()
     log(f"mask {w['dims']} porosity {flag.mean():.4f} in {time.perf_counter() - t0:.1f} s")
     # Practical ceiling: a streaming copy of 4 GiB (its rocprofv3 FETCH_SIZE /
     # WRITE_SIZE also calibrate the counter scale for the step kernel).
@@ -724,7 +727,8 @@ def main():
                    "layout": "AoSoA64" if shape["layout"] else "SoA",
                    "node_order": ["reference", "bricks", "pencils"][shape["ordering"]],
                    "brick": list(w.get("brick", (16, 4, 4))), "block": shape["block"],
-                   "waves_per_simd": shape["waves"], "placement_reserve_gib": w.get("reserve_gib", 0)},
+                   "waves_per_simd": shape["waves"], "placement_reserve_gib": w.get("reserve_gib", 0),
+                   "mask_seed": w["seed"], "flag_bin_sha256": mask_sha256},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": (f"{tname}, rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, measured at commit "
