@@ -397,8 +397,9 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0, tune_ste
     one after another (one sample per launch, same kernels and convergence
     rule); at most `waves` step-kernel waves per SIMD in both (TILE8, pencils
     of 32 rows, 256 threads).  Then the same per-step batch in the shape
-    plb_batch_autotune picks for the whole batch, and as one batch per size
-    class (L <= 128, L > 128), each in the shape tuned on that class."""
+    plb_batch_autotune picks for the whole batch, the persistent launch in the
+    shape tuned for it, and the per-step batch as one batch per size class
+    (L <= 128, L > 128), each in the shape tuned on that class."""
     flags = batch_samples(P, n_samples, seed)
     taus, gxs = [1.0] * n_samples, [1e-6] * n_samples
     kw = dict(precision=1, bb_mode=1, index_kind=1, layout=1, ordering=2, brick=(192, 32, 1), block_size=256,
@@ -435,6 +436,12 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0, tune_ste
     tuned = run_batch(everyone, tuned_cfg)
     tuned.pop("nodes")
     log(f"batch tuned {shape(tuned_cfg)}: {tuned['ms']:.1f} ms ({tune_s:.1f} s of tuning)")
+    t0 = time.perf_counter()
+    ptuned_cfg, _ = P.batch_autotune(flags, taus, gxs, tune_steps, cfg=base, persistent=True)
+    ptune_s = time.perf_counter() - t0
+    ptuned = run_batch(everyone, ptuned_cfg, persistent=True)
+    ptuned.pop("nodes")
+    log(f"batch persistent tuned {shape(ptuned_cfg)}: {ptuned['ms']:.1f} ms ({ptune_s:.1f} s of tuning)")
     classes = {"L<=128": [k for k in everyone if max(flags[k].shape) <= 128],
                "L>128": [k for k in everyone if max(flags[k].shape) > 128]}
     per_class = {"updates": 0, "ms": 0.0, "classes": {}}
@@ -483,6 +490,9 @@ def study_batch(P, n_samples, max_steps, check, tol, seed=100, waves=0, tune_ste
 
     out["tuned"] = dict(rate(tuned["upd"], tuned["ms"]), shape=shape(tuned_cfg), tuning_s=round(tune_s, 1),
                         same_steps=tuned["steps"] == per["steps"], K_max_rel_diff=kdiff(tuned["K"]))
+    out["persistent_tuned"] = dict(rate(ptuned["upd"], ptuned["ms"]), shape=shape(ptuned_cfg),
+                                   tuning_s=round(ptune_s, 1), same_steps=ptuned["steps"] == per["steps"],
+                                   K_max_rel_diff=kdiff(ptuned["K"]))
     out["tuned_per_class"] = dict(rate(per_class["updates"], per_class["ms"]), classes=per_class["classes"],
                                   same_steps=class_steps == per["steps"], K_max_rel_diff=kdiff(class_K))
     print(json.dumps(out), flush=True)

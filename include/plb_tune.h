@@ -44,9 +44,10 @@ int plb_autotune(const plb_config *base, const uint8_t *flag, int nx, int ny, in
                  int max_results, int *n_results);
 
 /* Launch-shape search for a batch of samples (plb_batch_create's inputs):
- * times `steps` fixed steps per candidate and writes the fastest
- * configuration to *best (precision, bounce-back and nodes per thread 1 kept
- * from base).  Stage 1: index encoding (I32, TILE8) x layout (SOA, AOSOA) x
+ * times `steps` fixed steps per candidate with plb_batch_run, or with
+ * plb_batch_run_persistent when `persistent` is nonzero, and writes the
+ * fastest configuration to *best (precision, bounce-back and nodes per
+ * thread 1 kept from base).  Stage 1: index encoding (I32, TILE8) x layout (SOA, AOSOA) x
  * node order (V order; pencils of 16, 32 and 64 full rows of each sample) at
  * base->block_size, one batch each.  Stage 2: block size 128, 256, 512 for
  * the winner.  Stage 3: waves per SIMD (no cap, 6, 5, 4), timed on one
@@ -55,7 +56,8 @@ int plb_autotune(const plb_config *base, const uint8_t *flag, int nx, int ny, in
  * sample sizes.  results (may be NULL) receives up to max_results timed
  * candidates. */
 int plb_batch_autotune(const plb_config *base, const plb_sample *samples, int n_samples, int steps,
-                       plb_config *best, plb_tune_result *results, int max_results, int *n_results);
+                       int persistent, plb_config *best, plb_tune_result *results, int max_results,
+                       int *n_results);
 
 #ifdef __cplusplus
 }

@@ -132,7 +132,7 @@ def _lib():
             "plb_mask_bcc": ([p, i32, d], i32),
             "plb_copy_bandwidth": ([i32, i64, i32, p], i32),
             "plb_autotune": ([p, p, i32, i32, i32, i32, i32, C.c_char_p, p, p, i32, p], i32),
-            "plb_batch_autotune": ([p, p, i32, i32, p, p, i32, p], i32),
+            "plb_batch_autotune": ([p, p, i32, i32, i32, p, p, i32, p], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -404,17 +404,18 @@ def _sample_array(flags, taus, gxs):
     return flags, arr
 
 
-def batch_autotune(flags, taus, gxs, steps=20, cfg=None, **kw):
+def batch_autotune(flags, taus, gxs, steps=20, cfg=None, persistent=False, **kw):
     """Fastest (index encoding, layout, node order, block size, waves per
     SIMD) for a batch of these samples, by plb_batch_autotune's three-stage
-    search; returns (Config, results)."""
+    search, timed with the per-step driver or (persistent=True) the
+    persistent launch; returns (Config, results)."""
     cfg = cfg or default_config(**kw)
     keep, arr = _sample_array(flags, taus, gxs)
     best = Config()
     res = (TuneResult * 64)()
     n = C.c_int(0)
-    _check(_lib().plb_batch_autotune(C.byref(cfg), arr, len(keep), steps, C.byref(best), res, 64, C.byref(n)),
-           "plb_batch_autotune")
+    _check(_lib().plb_batch_autotune(C.byref(cfg), arr, len(keep), steps, int(bool(persistent)), C.byref(best),
+                                     res, 64, C.byref(n)), "plb_batch_autotune")
     return best, _tune_rows(res[:n.value])
 
 

@@ -546,9 +546,12 @@ extern "C" int plb_batch_reduce(plb_batch* b, plb_reduction* out)
 // stage 2 one batch per block size for the winner; stage 3 times the wave
 // caps on the stage-2 winner's batch, since they change only the launch.
 extern "C" int plb_batch_autotune(const plb_config* base, const plb_sample* samples, int n_samples, int steps,
-                                  plb_config* best, plb_tune_result* results, int max_results, int* n_results)
+                                  int persistent, plb_config* best, plb_tune_result* results, int max_results,
+                                  int* n_results)
 {
     if (!base || !samples || n_samples < 1 || !best || steps < 1 || max_results < 0) return PLB_ERR_INVALID;
+    if (persistent && n_samples > PLB_PERSIST_MAX_SAMPLES) return PLB_ERR_INVALID;
+    auto run = persistent ? plb_batch_run_persistent : plb_batch_run;
     Range range("plb_batch_autotune");
     int rc = check_cfg(base);
     if (rc) return rc;
@@ -575,8 +578,8 @@ extern "C" int plb_batch_autotune(const plb_config* base, const plb_sample* samp
         if (r) return r;
         int64_t upd = 0;
         double ms = 0.0;
-        r = plb_batch_run(b, std::max(2, steps / 4), steps, 0.0, nullptr, &upd, &ms);
-        if (!r) r = plb_batch_run(b, steps, steps, 0.0, nullptr, &upd, &ms);
+        r = run(b, std::max(2, steps / 4), steps, 0.0, nullptr, &upd, &ms);
+        if (!r) r = run(b, steps, steps, 0.0, nullptr, &upd, &ms);
         if (!r) record(cfg, upd, ms);
         if (r || !keep)
             plb_batch_destroy(b);
@@ -624,7 +627,7 @@ extern "C" int plb_batch_autotune(const plb_config* base, const plb_sample* samp
         keep->c->cfg.waves_per_simd = waves;
         int64_t upd = 0;
         double ms = 0.0;
-        rc = plb_batch_run(keep, steps, steps, 0.0, nullptr, &upd, &ms);
+        rc = run(keep, steps, steps, 0.0, nullptr, &upd, &ms);
         if (rc) break;
         record(cfg, upd, ms);
     }

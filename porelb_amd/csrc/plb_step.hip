@@ -630,19 +630,8 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
         __syncthreads();
     };
     rebuild();
-    // Pacing: a block starts its r-th chunk of a step no earlier than r
-    // rounds after the step began, a round being 31/32 of the previous plain
-    // step's duration per round.  Blocks that run ahead wait, so the blocks
-    // of an XCD keep sweeping neighbouring chunks together, as the hardware
-    // dispatcher keeps them in a launch per step.
-    uint64_t t_prev = 0, tau = 0;
-    int64_t rounds_prev = 0;
-    bool prev_plain = false;
     for (int64_t t = 0; t < p.max_steps; t++) {
         const bool check = (t + 1) % p.check_every == 0 || t + 1 == p.max_steps;
-        const uint64_t t_start = wall_clock64();
-        if (prev_plain && rounds_prev > 0) tau = (t_start - t_prev) * 31 / 32 / (uint64_t)rounds_prev;
-        t_prev = t_start;
         const int par = (p.cur0 + (int)(t & 1)) & 1;
         a.fin = p.f[par];
         a.fout = p.f[par ^ 1];
@@ -651,10 +640,7 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
         const int64_t total = aprefix[p.nsamples];
         const int64_t j0 = grp * total / 8, j1 = (grp + 1) * total / 8;
         int s = 0;
-        int64_t r = 0;
-        for (int64_t j = j0 + slot; j < j1; j += grp_blocks, r++) {
-            if (tau)
-                while (wall_clock64() - t_start < (uint64_t)r * tau) __builtin_amdgcn_s_sleep(1);
+        for (int64_t j = j0 + slot; j < j1; j += grp_blocks) {
             while (aprefix[s + 1] <= j) s++;   // uniform per block; j only grows
             const int64_t chunk = p.sample_off[s] / bs + (j - aprefix[s]);
             const int64_t i = chunk * bs + threadIdx.x;
@@ -669,8 +655,6 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
                 if (threadIdx.x == 0) p.partial[chunk] = t_sum;
             }
         }
-        rounds_prev = r;
-        prev_plain = !check;
         if (!grid_barrier(p, gen)) return;
         // Density guard: stop after the faulting step.  Only a fault of this
         // step or earlier counts: a faster block may already be in the next

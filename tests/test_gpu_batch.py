@@ -54,14 +54,16 @@ def test_batch_convergence_stops_per_sample(plb):
             assert np.array_equal(b.read_macros(s), lat.read_macros()), s
 
 
-def test_batch_autotune_shape_keeps_results(plb):
-    """plb_batch_autotune: 16 stage-1 batches (2 index kinds x 2 layouts x 4
-    node orders), 2 other block sizes, the winner again and 3 wave caps; the
-    shape it picks gives every sample the macros of the default shape."""
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_autotune_shape_keeps_results(plb, persistent):
+    """plb_batch_autotune, timing either driver: 16 stage-1 batches (2 index
+    kinds x 2 layouts x 4 node orders), 2 other block sizes, the winner again
+    and 3 wave caps; the shape it picks gives every sample the macros of the
+    default shape."""
     fl = masks(plb)
     taus, gxs = [0.8, 1.0, 0.6], [1e-5, 2e-5, 5e-6]
     base = plb.default_config(precision=1, bb_mode=plb.BB_HALFWAY, block_size=256)
-    best, res = plb.batch_autotune(fl, taus, gxs, steps=8, cfg=base)
+    best, res = plb.batch_autotune(fl, taus, gxs, steps=8, cfg=base, persistent=persistent)
     assert len(res) == 16 + 2 + 1 + 3 and all(r["mlups"] > 0 for r in res)
     assert {(r["index_kind"], r["layout"]) for r in res[:16]} == {(0, 0), (0, 1), (1, 0), (1, 1)}
     assert {r["block"] for r in res[16:18]} == {128, 512}
@@ -74,7 +76,7 @@ def test_batch_autotune_shape_keeps_results(plb):
     for cfg in (base, best):
         b = plb.Batch(fl, taus, gxs, cfg=cfg)
         try:
-            b.run(120, 40, 0.0)
+            b.run(120, 40, 0.0, persistent=persistent)
             out.append([b.read_macros(s) for s in range(len(fl))])
         finally:
             b.close()
