@@ -366,7 +366,7 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
         act[s] = b->n[s] > 0 ? 1 : 0;
         n_active += act[s];
     }
-    // per-run device state: active, cur, steps, prev, n_active, abort, barrier counter
+    // per-run device state: active, cur, steps, prev, n_active, abort, barrier and claim counters
     DevBuf<int> d_act, d_cur, d_flags;
     DevBuf<int64_t> d_steps;
     DevBuf<double> d_prev;
@@ -376,14 +376,14 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     PLB_HIP(d_steps.alloc(S));
     PLB_HIP(d_prev.alloc(S));
     PLB_HIP(d_flags.alloc(2));
-    PLB_HIP(d_bar.alloc(17 * 16));
+    PLB_HIP(d_bar.alloc(34 * 16));
     const int flags[2] = {n_active, 0};
     PLB_HIP(hipMemcpyAsync(d_act, act.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_cur, cur.data(), sizeof(int) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_steps, steps.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_prev, prev.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
     PLB_HIP(hipMemcpyAsync(d_flags, flags, sizeof flags, hipMemcpyHostToDevice, c->stream));
-    PLB_HIP(hipMemsetAsync(d_bar, 0, sizeof(unsigned long long) * 17 * 16, c->stream));
+    PLB_HIP(hipMemsetAsync(d_bar, 0, sizeof(unsigned long long) * 34 * 16, c->stream));
     PersistArgs p;
     p.f[0] = c->f[0];
     p.f[1] = c->f[1];
@@ -411,25 +411,38 @@ extern "C" int plb_batch_run_persistent(plb_batch* b, int64_t max_steps, int64_t
     PLB_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device));
     if (clock_khz < 1000) clock_khz = 100000;  // no usable rate reported: the 100 MHz of gfx9
     p.wait_ticks = (uint64_t)clock_khz * 2000ull;  // 2 s of wall_clock64
-    int blocks = 0;
-    PLB_HIP(hipEventRecord(c->ev0, c->stream));
-    if (n_active > 0 && max_steps > 0) {
-        rc = launch_batch_persistent(c, p, &blocks);
-        if (rc) return rc;
-    }
-    PLB_HIP(hipEventRecord(c->ev1, c->stream));
-    PLB_HIP(hipEventSynchronize(c->ev1));
-    float t = 0.f;
-    PLB_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    p.start_ticks = (uint64_t)clock_khz * 50ull;   // 50 ms
+    int cus = 0;
+    PLB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    int blocks = 0, cap = 0;
     int hflags[2] = {0, 0};
+    float t = 0.f;
+    // A grid that is not all resident leaves before any work with abort = 2
+    // (the occupancy figure is an estimate); it is launched again with one
+    // block fewer per CU, with the barrier state cleared.
+    for (;;) {
+        PLB_HIP(hipEventRecord(c->ev0, c->stream));
+        if (n_active > 0 && max_steps > 0) {
+            rc = launch_batch_persistent(c, p, cap, &blocks);
+            if (rc) return rc;
+        }
+        PLB_HIP(hipEventRecord(c->ev1, c->stream));
+        PLB_HIP(hipEventSynchronize(c->ev1));
+        PLB_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+        PLB_HIP(hipMemcpy(hflags, d_flags, sizeof hflags, hipMemcpyDeviceToHost));
+        if (hflags[1] != 2 || blocks <= cus) break;
+        cap = blocks - cus;
+        PLB_HIP(hipMemset(d_bar, 0, sizeof(unsigned long long) * 34 * 16));
+        PLB_HIP(hipMemcpy(d_flags, flags, sizeof flags, hipMemcpyHostToDevice));
+    }
     unsigned long long key = NO_FAULT;
-    PLB_HIP(hipMemcpy(hflags, d_flags, sizeof hflags, hipMemcpyDeviceToHost));
     PLB_HIP(hipMemcpy(&key, c->fault, sizeof key, hipMemcpyDeviceToHost));
     PLB_HIP(hipMemcpy(cur.data(), d_cur, sizeof(int) * S, hipMemcpyDeviceToHost));
     PLB_HIP(hipMemcpy(steps.data(), d_steps, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
     if (hflags[1]) {
         fprintf(stderr,
-                "libplb: persistent batch run: a grid barrier wait exceeded 2 s (%d blocks, wall clock %d kHz)\n",
+                hflags[1] == 2 ? "libplb: persistent batch run: the grid did not start whole (%d blocks, wall clock %d kHz)\n"
+                               : "libplb: persistent batch run: a grid barrier wait exceeded 2 s (%d blocks, wall clock %d kHz)\n",
                 blocks, clock_khz);
         return PLB_ERR_HIP;
     }

@@ -209,12 +209,14 @@ struct PersistArgs {
     int64_t* steps;               // per sample: steps run
     double* prev;                 // per sample: <u_x> at the previous check (NaN before)
     double* partial;              // per chunk: u_x sum on check steps
-    unsigned long long* bar;      // grid-barrier counters: 17 lines of 16 words, zeroed
-    int* abort;                   // a barrier wait timed out
+    unsigned long long* bar;      // grid-barrier, chunk-claim and start counters: 34 lines of 16 words, zeroed
+    int* abort;                   // 1: a barrier wait timed out; 2: the grid was not all resident
     int* n_active;                // samples still running
     uint64_t wait_ticks;          // wall_clock64 ticks a barrier wait may take
+    uint64_t start_ticks;         // ticks the first block waits for the whole grid to start
 };
-int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int* blocks);
+// max_blocks > 0 caps the grid below the computed residency.
+int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int max_blocks, int* blocks);
 int build_table_from_V(plb_ctx* c, const plb_node_info* V, int64_t n);
 int build_table_from_flag(plb_ctx* c, const uint8_t* flag);
 int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_base = nullptr);

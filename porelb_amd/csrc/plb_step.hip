@@ -154,7 +154,18 @@ __device__ __forceinline__ int32_t bitmap_lookup(uint2 wd, uint32_t v)
     return ((wd.x >> b) & 1u) ? id : TAB_BOUNCE;
 }
 
-template <int LAYOUT, int IDX>
+// A read through the constant address space, for data no kernel of the
+// launch writes (sample tables, pull-table descriptors): it stays a scalar
+// load where the address is uniform, also in kernels that store elsewhere.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* q)
+{
+    return *(const __attribute__((address_space(4))) T*)q;
+}
+
+// CONST_DESC: read the TILE8 descriptors through the constant address space
+// (for loops that also store; one-node kernels get scalar loads anyway).
+template <int LAYOUT, int IDX, bool CONST_DESC = false>
 __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entries<IDX>& e)
 {
     // Index words are read once per step: non-temporal loads keep them from
@@ -174,8 +185,15 @@ __device__ __forceinline__ void issue_entries(const StepArgs& a, int64_t i, Entr
         for (int k = 0; k < TILE8_FULL; k++) e.w[k] = __builtin_nontemporal_load(o + k * 64);
         e.w[TILE8_FULL] = __builtin_nontemporal_load(a.toff16 + tile * 64 + (i & 63));
         const int32_t* d = a.tdesc + tile * (Q - 1);
+        if constexpr (CONST_DESC) {
+            // never written while steps run: scalar loads even in a loop
+            // that stores, where the compiler could not prove it otherwise
 #pragma unroll
-        for (int j = 1; j < Q; j++) e.desc[j] = d[j - 1];
+            for (int j = 1; j < Q; j++) e.desc[j] = ld_const(d + j - 1);
+        } else {
+#pragma unroll
+            for (int j = 1; j < Q; j++) e.desc[j] = d[j - 1];
+        }
     }
 }
 
@@ -434,11 +452,11 @@ __device__ __forceinline__ void collide_any(const StepArgs& a, int64_t i, const 
         collide_shift<S, LAYOUT>(a, i, t, raw, fk, key);
 }
 
-template <typename S, int LAYOUT, int IDX>
+template <typename S, int LAYOUT, int IDX, bool CONST_DESC = false>
 __device__ __forceinline__ void node_any(const StepArgs& a, int64_t i, unsigned long long fk)
 {
     Entries<IDX> e;
-    issue_entries<LAYOUT, IDX>(a, i, e);
+    issue_entries<LAYOUT, IDX, CONST_DESC>(a, i, e);
     const uint32_t key = sr_key<S>(a, i);
     decode_entries<IDX>(a, i, e);
     S raw[Q];
@@ -561,6 +579,7 @@ __device__ __forceinline__ double wave_sum_chunks(const double* partial, int64_t
 // without a memory round trip.
 constexpr int PERSIST_MAX_SAMPLES = PLB_PERSIST_MAX_SAMPLES;
 constexpr int BAR_STRIDE = 16;  // u64 words between counters: one 128-byte line each
+constexpr int PERSIST_CLAIM = 4;  // chunks a block claims at a time
 
 __device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long long& gen)
 {
@@ -611,13 +630,46 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
     // iff aprefix[s + 1] > aprefix[s]); rebuilt after every decision barrier.
     __shared__ int aprefix[PERSIST_MAX_SAMPLES + 1];
     __shared__ int stop;
+    __shared__ int64_t claim;   // this block's claimed chunk offset in its stretch
     const int bs = blockDim.x;
     // The active chunks, in sample order, are split into eight contiguous
-    // stretches, one per XCD (block id mod 8); the blocks of a stretch take
-    // its chunks in turn, so the XCD sweeps through its stretch in order.
+    // stretches, one per XCD (block id mod 8).  The blocks of a stretch claim
+    // its chunks CLAIM at a time from the stretch's counter for this step
+    // (line 17 + 2 * group + step parity of p.bar), so the XCD sweeps its
+    // stretch in order and a block that finishes early takes more chunks,
+    // as the hardware dispatcher does in a launch per step.  Each block
+    // claims its next chunks before it processes the current ones, which
+    // hides the atomic's round trip.
     const unsigned G = gridDim.x, grp = blockIdx.x & 7u, slot = blockIdx.x >> 3;
-    const int64_t grp_blocks = (G - 1 - grp) / 8 + 1;
+    const int64_t ngrp = G < 8 ? G : 8;   // groups that have blocks
     unsigned long long gen = 0;
+    // Residency check before any work: every block counts itself in and
+    // waits, briefly, for the whole grid.  A grid larger than what the CUs
+    // really hold never completes the count; the blocks then leave with
+    // abort = 2 and the host launches a smaller grid.
+    if (threadIdx.x == 0) {
+        int good = 0;
+        unsigned long long* started = p.bar + 33 * BAR_STRIDE;
+        if (!__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_fetch_add(started, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t0 = wall_clock64();
+            for (;;) {
+                if (__hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) {
+                    good = 1;
+                    break;
+                }
+                if (__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (wall_clock64() - t0 > p.start_ticks) {
+                    __hip_atomic_store(p.abort, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        stop = !good;
+    }
+    __syncthreads();
+    if (stop) return;
     auto rebuild = [&]() {
         if (threadIdx.x == 0) {
             int acc = 0;
@@ -638,22 +690,44 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
         a.it = t;
         a.write_macros = check ? 1 : 0;
         const int64_t total = aprefix[p.nsamples];
-        const int64_t j0 = grp * total / 8, j1 = (grp + 1) * total / 8;
+        const int64_t j0 = grp * total / ngrp, j1 = (grp + 1) * total / ngrp;
+        unsigned long long* claim_cnt = p.bar + (17 + 2 * grp + (t & 1)) * BAR_STRIDE;
+        // This step's counter was zeroed in the step before (or by the host);
+        // the other parity's is zeroed now for the next step, by one block of
+        // the group: the barrier at the end of this step orders that store
+        // before any claim of the next step.
+        if (slot == 0 && threadIdx.x == 0)
+            __hip_atomic_store(p.bar + (17 + 2 * grp + ((t + 1) & 1)) * BAR_STRIDE, 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0)
+            claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, (unsigned long long)PERSIST_CLAIM, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        int64_t mine = claim;
         int s = 0;
-        for (int64_t j = j0 + slot; j < j1; j += grp_blocks) {
-            while (aprefix[s + 1] <= j) s++;   // uniform per block; j only grows
-            const int64_t chunk = p.sample_off[s] / bs + (j - aprefix[s]);
-            const int64_t i = chunk * bs + threadIdx.x;
-            const bool live = i < p.sample_end[s];
-            if (live) {
-                a.omega = p.omega[s];
-                a.gx = p.gx[s];
-                node_any<S, LAYOUT, IDX>(a, i, NO_FAULT);
+        while (j0 + mine < j1) {
+            __syncthreads();   // every thread has read `claim` before it is replaced
+            if (threadIdx.x == 0)
+                claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, (unsigned long long)PERSIST_CLAIM,
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int64_t jend = j0 + mine + PERSIST_CLAIM < j1 ? j0 + mine + PERSIST_CLAIM : j1;
+            for (int64_t j = j0 + mine; j < jend; j++) {
+                while (aprefix[s + 1] <= j) s++;   // uniform per block; j only grows
+                const int64_t chunk = ld_const(p.sample_off + s) / bs + (j - aprefix[s]);
+                const int64_t i = chunk * bs + threadIdx.x;
+                const bool live = i < ld_const(p.sample_end + s);
+                if (live) {
+                    a.omega = ld_const(p.omega + s);
+                    a.gx = ld_const(p.gx + s);
+                    node_any<S, LAYOUT, IDX, true>(a, i, NO_FAULT);
+                }
+                if (check) {
+                    const double t_sum = block_sum_fixed(live ? a.macros[a.Npad + i] : 0.0, lds);
+                    if (threadIdx.x == 0) p.partial[chunk] = t_sum;
+                }
             }
-            if (check) {
-                const double t_sum = block_sum_fixed(live ? a.macros[a.Npad + i] : 0.0, lds);
-                if (threadIdx.x == 0) p.partial[chunk] = t_sum;
-            }
+            __syncthreads();   // the next claim has landed in LDS
+            mine = claim;
         }
         if (!grid_barrier(p, gen)) return;
         // Density guard: stop after the faulting step.  Only a fault of this
@@ -1115,7 +1189,7 @@ int launch_step_batch(plb_ctx* c, int64_t it, bool write_macros, const int32_t* 
 // The persistent batch run as one cooperative launch of as many blocks as can
 // be resident at once (at most one per chunk).  *blocks = the grid size.
 template <typename S, int LAYOUT, int IDX>
-static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int* blocks)
+static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int max_blocks, int* blocks)
 {
     auto kern = k_batch_persistent<S, LAYOUT, IDX>;
     const int bs = c->cfg.block_size;
@@ -1131,7 +1205,8 @@ static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int* blocks)
     // computed limit waited at its first barrier for blocks that never
     // started (the limit is an estimate; the bounded wait reported it).
     if (per_cu > 1) per_cu--;
-    const int64_t g = std::min<int64_t>((int64_t)per_cu * cus, p.nchunks);
+    int64_t g = std::min<int64_t>((int64_t)per_cu * cus, p.nchunks);
+    if (max_blocks > 0) g = std::min<int64_t>(g, max_blocks);
     if (g < 1) return PLB_ERR_INVALID;
     StepArgs a = make_args(c, 0, false);
     PersistArgs pp = p;
@@ -1141,10 +1216,10 @@ static int launch_persistent_t(plb_ctx* c, const PersistArgs& p, int* blocks)
     return PLB_OK;
 }
 
-int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int* blocks)
+int launch_batch_persistent(plb_ctx* c, const PersistArgs& p, int max_blocks, int* blocks)
 {
     const int pr = c->cfg.precision, lay = c->cfg.layout, idx = c->cfg.index_kind;
-#define PLB_PERSIST(ST, L, I) if (pr == (sizeof(ST) == 8 ? PLB_FP64 : sizeof(ST) == 4 ? PLB_FP32 : PLB_FP16) && lay == L && idx == I) return launch_persistent_t<ST, L, I>(c, p, blocks)
+#define PLB_PERSIST(ST, L, I) if (pr == (sizeof(ST) == 8 ? PLB_FP64 : sizeof(ST) == 4 ? PLB_FP32 : PLB_FP16) && lay == L && idx == I) return launch_persistent_t<ST, L, I>(c, p, max_blocks, blocks)
     PLB_PERSIST(double, PLB_LAYOUT_SOA, PLB_INDEX_I32);
     PLB_PERSIST(double, PLB_LAYOUT_SOA, PLB_INDEX_TILE8);
     PLB_PERSIST(double, PLB_LAYOUT_AOSOA, PLB_INDEX_I32);
