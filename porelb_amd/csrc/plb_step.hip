@@ -579,7 +579,6 @@ __device__ __forceinline__ double wave_sum_chunks(const double* partial, int64_t
 // without a memory round trip.
 constexpr int PERSIST_MAX_SAMPLES = PLB_PERSIST_MAX_SAMPLES;
 constexpr int BAR_STRIDE = 16;  // u64 words between counters: one 128-byte line each
-constexpr int PERSIST_CLAIM = 4;  // chunks a block claims at a time
 
 __device__ __forceinline__ bool grid_barrier(const PersistArgs& p, unsigned long long& gen)
 {
@@ -630,19 +629,17 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
     // iff aprefix[s + 1] > aprefix[s]); rebuilt after every decision barrier.
     __shared__ int aprefix[PERSIST_MAX_SAMPLES + 1];
     __shared__ int stop;
-    __shared__ int64_t claim;   // this block's claim number in its stretch
+    __shared__ int64_t claim;   // the chunk this block claimed next, in its stretch
     const int bs = blockDim.x;
     // The active chunks, in sample order, are split into eight contiguous
     // stretches, one per XCD (block id mod 8).  The blocks of a stretch take
-    // claim numbers k = 0, 1, ... from the stretch's counter for this step
-    // (line 17 + 2 * group + step parity of p.bar).  With B blocks in the
-    // group, claim k covers the CLAIM chunks (k / B) * CLAIM * B + k % B +
-    // m * B, m = 0 .. CLAIM-1: one atomic per CLAIM chunks, while the chunks
-    // in flight stay about B consecutive ones, so the XCD sweeps its stretch
-    // in order and a block that finishes early takes more, as the hardware
-    // dispatcher does in a launch per step.  Each block claims its next
-    // chunks before it processes the current ones, which hides the atomic's
-    // round trip.
+    // its chunks one at a time from the stretch's counter for this step
+    // (line 17 + 2 * group + step parity of p.bar), so the XCD sweeps its
+    // stretch in order and a block that finishes early takes more, as the
+    // hardware dispatcher does in a launch per step.  Each block claims its
+    // next chunk before it processes the current one, which hides the
+    // atomic's round trip.  (Claims of 2, 4 or 8 chunks, spread so that the
+    // chunks in flight stay consecutive, were slower.)
     const unsigned G = gridDim.x, grp = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const int64_t ngrp = G < 8 ? G : 8;   // groups that have blocks
     unsigned long long gen = 0;
@@ -702,39 +699,29 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
         if (slot == 0 && threadIdx.x == 0)
             __hip_atomic_store(p.bar + (17 + 2 * grp + ((t + 1) & 1)) * BAR_STRIDE, 0ull, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        const int64_t len = j1 - j0, B = (G - 1 - grp) / ngrp + 1;
+        const int64_t len = j1 - j0;
         if (threadIdx.x == 0)
             claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         int64_t mine = claim;
-        for (;;) {
-            const int64_t first = mine / B * (PERSIST_CLAIM * B) + mine % B;   // grows with the claim number
-            if (first >= len) break;
+        int s = 0;
+        while (mine < len) {
             __syncthreads();   // every thread has read the claim before it is replaced
             if (threadIdx.x == 0)
                 claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (int m = 0; m < PERSIST_CLAIM; m++) {
-                const int64_t j = j0 + first + m * B;
-                if (j >= j1) break;
-                // the sample of active chunk j: the last s with aprefix[s] <= j
-                int lo = 0, hi = p.nsamples;   // aprefix[lo] <= j < aprefix[hi]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (aprefix[mid] <= j) lo = mid; else hi = mid;
-                }
-                const int s = lo;
-                const int64_t chunk = ld_const(p.sample_off + s) / bs + (j - aprefix[s]);
-                const int64_t i = chunk * bs + threadIdx.x;
-                const bool live = i < ld_const(p.sample_end + s);
-                if (live) {
-                    a.omega = ld_const(p.omega + s);
-                    a.gx = ld_const(p.gx + s);
-                    node_any<S, LAYOUT, IDX, true>(a, i, NO_FAULT);
-                }
-                if (check) {
-                    const double t_sum = block_sum_fixed(live ? a.macros[a.Npad + i] : 0.0, lds);
-                    if (threadIdx.x == 0) p.partial[chunk] = t_sum;
-                }
+            const int64_t j = j0 + mine;
+            while (aprefix[s + 1] <= j) s++;   // uniform per block; a block's claims only grow
+            const int64_t chunk = ld_const(p.sample_off + s) / bs + (j - aprefix[s]);
+            const int64_t i = chunk * bs + threadIdx.x;
+            const bool live = i < ld_const(p.sample_end + s);
+            if (live) {
+                a.omega = ld_const(p.omega + s);
+                a.gx = ld_const(p.gx + s);
+                node_any<S, LAYOUT, IDX, true>(a, i, NO_FAULT);
+            }
+            if (check) {
+                const double t_sum = block_sum_fixed(live ? a.macros[a.Npad + i] : 0.0, lds);
+                if (threadIdx.x == 0) p.partial[chunk] = t_sum;
             }
             __syncthreads();   // the next claim has landed in LDS
             mine = claim;
