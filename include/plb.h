@@ -302,9 +302,12 @@ int plb_batch_run(plb_batch *b, int64_t max_steps, int64_t check_every, double t
  * the per-sample u_x sums and stop decisions made on the device (no host
  * round trip per check).  The per-sample sums use the same fixed order in
  * both drivers, so both stop every sample at the same step and leave the
- * same states.  A barrier wait longer than 2 s ends the launch and returns
- * PLB_ERR_HIP.  At most PLB_PERSIST_MAX_SAMPLES samples (PLB_ERR_INVALID
- * beyond). */
+ * same states.  The blocks of each XCD take the chunks of their share of
+ * the batch one at a time from a device counter.  A grid that turns out not
+ * to be all resident leaves before any work and is launched again with one
+ * block fewer per CU; a barrier wait longer than 2 s ends the launch and
+ * returns PLB_ERR_HIP.  ms covers the last launch.  At most
+ * PLB_PERSIST_MAX_SAMPLES samples (PLB_ERR_INVALID beyond). */
 #define PLB_PERSIST_MAX_SAMPLES 2048
 int plb_batch_run_persistent(plb_batch *b, int64_t max_steps, int64_t check_every, double tol,
                              int64_t *steps_done, int64_t *updates, double *ms);
