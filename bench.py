@@ -518,6 +518,36 @@ def study_batch_steps(P, steps, waves=4, seed=100):
                       "persistent": {"ms": round(m2, 1), "mlups": round(u2 / (m2 * 1e-3) / 1e6, 1)}}), flush=True)
 
 
+def study_aa(P, steps=100, warmup=10):
+    """The 1024^3 sphere pack in both stream modes: two buffers with pull
+    steps, and one buffer with the AA pattern (PLB_STREAM_AA: local and
+    neighbour steps alternating), FP32 and FP16 storage in their benchmark
+    shapes, `steps` timed steps each after `warmup` (an even count, so both
+    AA kernels run equally often), two rounds in alternation."""
+    w = SPHERES1024
+    flag = make_mask(P, w)
+    out = {"study": "aa", "workload": w["label"], "steps": steps, "runs": []}
+    for name in ("spheres1024-fp32", "spheres1024-fp16"):
+        wk = WORKLOADS[name]
+        for mode in (0, 1, 0, 1):
+            cfg = P.default_config(tau=w["tau"], gx=w["gx"], precision=wk["precision"], bb_mode=1,
+                                   index_kind=wk["index"], layout=wk["layout"], ordering=wk["ordering"],
+                                   brick=wk["brick"], block_size=wk["block"], waves_per_simd=wk.get("waves", 0),
+                                   fp16_rounding=wk.get("fp16_rounding", 1), placement_reserve_gib=w["reserve_gib"],
+                                   stream_mode=mode)
+            lat = P.Lattice.from_flag(flag, 1, cfg=cfg)
+            try:
+                lat.step(0, warmup)
+                ms = lat.step_timed(warmup, steps)
+                run = {"workload": name, "stream": "aa" if mode else "pull", "mlups": round(lat.N * steps / ms / 1e3, 1),
+                       "ms_per_step": round(ms / steps, 4), "bytes_per_update": round(lat.bytes_per_update(), 2)}
+            finally:
+                lat.close()
+            log(json.dumps(run))
+            out["runs"].append(run)
+    print(json.dumps(out), flush=True)
+
+
 def study_placement(P, reserve_gib, steps=200, warmup=20):
     """Which allocations of the FP32 1024^3 lattice stream slower: after the
     benchmark's mask and copy-bandwidth run, hold `reserve_gib` GiB, then time
@@ -571,7 +601,7 @@ def main():
     ap.add_argument("--index", type=int, default=None)
     ap.add_argument("--waves", type=int, default=None, help="step-kernel waves per SIMD (0 = no cap)")
     ap.add_argument("--study", choices=["precision", "batch", "batch-steps", "config4", "preprocess", "outputs",
-                                        "placement"],
+                                        "placement", "aa"],
                     default=None,
                     help="run a study instead of the benchmark line")
     ap.add_argument("--max-steps", type=int, default=100000)
@@ -595,6 +625,9 @@ def main():
     if args.study == "batch":
         study_batch(P, args.samples, args.max_steps, args.check, args.tol if args.tol is not None else 1e-6,
                     waves=args.waves if args.waves is not None else 5)
+        return
+    if args.study == "aa":
+        study_aa(P)
         return
     if args.study == "batch-steps":
         study_batch_steps(P, args.max_steps if args.max_steps != 100000 else 200)

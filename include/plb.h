@@ -121,8 +121,17 @@ typedef struct {
                             first process allocates stream about 6 % slower
                             for as long as they live; 40 places a lattice
                             clear of them.  0: no reserve                    */
+    int stream_mode;     /* PLB_STREAM_PULL: two population buffers, every
+                            step pulls through the table.  PLB_STREAM_AA: one
+                            buffer, AA pattern -- even steps update each node
+                            in place (no table read), odd steps gather from
+                            and scatter to the neighbours; the same results,
+                            half the population memory.  AA needs half-way
+                            bounce-back, nodes_per_thread 1, graph_steps 0
+                            and a single lattice (not a batch)               */
 } plb_config;
 enum { PLB_ROUND_NEAREST = 0, PLB_ROUND_STOCHASTIC = 1 };
+enum { PLB_STREAM_PULL = 0, PLB_STREAM_AA = 1 };
 
 /* Fills the reference's parameters (porelb.c:83-94) and FP64/REFCOMPAT/SOA/
  * REFERENCE order/I32 index, 256 threads, one node per thread, device 0. */

@@ -31,6 +31,7 @@ LAYOUT_SOA, LAYOUT_AOSOA = 0, 1
 ORDER_REFERENCE, ORDER_BRICK, ORDER_PENCIL = 0, 1, 2
 INDEX_I32, INDEX_TILE8, INDEX_BITMAP = 0, 1, 2
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+STREAM_PULL, STREAM_AA = 0, 1
 NOALLOC_MAX_NODES = 1 << 20     # PLB_NOALLOC_MAX_NODES
 TYPE_FLUID, TYPE_SOLID, TYPE_EXTERN = 0, 1, 2
 ERR_DENSITY = -6
@@ -54,7 +55,8 @@ class Config(C.Structure):
                 ("ordering", C.c_int), ("brick", C.c_int * 3), ("block_size", C.c_int),
                 ("device", C.c_int), ("fp16_scale_log2", C.c_int), ("index_kind", C.c_int),
                 ("nodes_per_thread", C.c_int), ("graph_steps", C.c_int), ("waves_per_simd", C.c_int),
-                ("fp16_rounding", C.c_int), ("placement_reserve_gib", C.c_int)]
+                ("fp16_rounding", C.c_int), ("placement_reserve_gib", C.c_int),
+                ("stream_mode", C.c_int)]
 
 
 class Reduction(C.Structure):

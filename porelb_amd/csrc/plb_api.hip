@@ -44,6 +44,7 @@ extern "C" void plb_config_default(plb_config* cfg)
     cfg->graph_steps = 0;
     cfg->waves_per_simd = 0;
     cfg->fp16_rounding = PLB_ROUND_NEAREST;
+    cfg->stream_mode = PLB_STREAM_PULL;
 }
 
 extern "C" const char* plb_strerror(int code)
@@ -100,6 +101,10 @@ int plb::check_cfg(const plb_config* cfg)
     if (cfg->fp16_rounding != PLB_ROUND_NEAREST && cfg->fp16_rounding != PLB_ROUND_STOCHASTIC)
         return PLB_ERR_INVALID;
     if (cfg->placement_reserve_gib < 0 || cfg->placement_reserve_gib > 160) return PLB_ERR_INVALID;
+    if (cfg->stream_mode != PLB_STREAM_PULL && cfg->stream_mode != PLB_STREAM_AA) return PLB_ERR_INVALID;
+    if (cfg->stream_mode == PLB_STREAM_AA &&
+        (cfg->bb_mode != PLB_BB_HALFWAY || cfg->nodes_per_thread != 1 || cfg->graph_steps != 0))
+        return PLB_ERR_INVALID;
     return PLB_OK;
 }
 
@@ -146,7 +151,10 @@ int plb::ctx_alloc_state(plb_ctx* c)
 {
     const size_t nf = (size_t)c->Npad * Q * c->elem;
     PLB_HIP(hipMalloc(&c->f[0], nf));
-    PLB_HIP(hipMalloc(&c->f[1], nf));
+    if (c->cfg.stream_mode == PLB_STREAM_AA)
+        c->f[1] = c->f[0];  // one buffer; c->cur is the AA phase
+    else
+        PLB_HIP(hipMalloc(&c->f[1], nf));
     PLB_HIP(hipMalloc(&c->macros, sizeof(double) * 4 * c->Npad));
     PLB_HIP(hipMemsetAsync(c->macros, 0, sizeof(double) * 4 * c->Npad, c->stream));
     PLB_HIP(hipMalloc(&c->fault, sizeof(unsigned long long)));
@@ -167,7 +175,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
     }
-    void* ptrs[] = {c->f[0], c->f[1], c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
+    void* ptrs[] = {c->f[0], c->f[1] != c->f[0] ? c->f[1] : nullptr, c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
                     c->inv_perm, c->key_perm, c->node_xyz, c->macros, c->fault, c->red, c->reserve};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -256,7 +264,9 @@ extern "C" int plb_init_equilibrium(plb_ctx* c)
 {
     if (!c) return PLB_ERR_INVALID;
     PLB_HIP(hipSetDevice(c->device));
-    c->cur = 0;
+    // AA: the rest state (every slot w_j rho0) is the local-step form, so the
+    // first step is a neighbour step, like a pull step
+    c->cur = c->cfg.stream_mode == PLB_STREAM_AA ? 1 : 0;
     int rc = launch_init_eq(c);
     if (rc) return rc;
     hipLaunchKernelGGL(k_fill_macros, dim3((unsigned)((c->Npad + 255) / 256)), dim3(256), 0,
@@ -379,10 +389,13 @@ extern "C" double plb_bytes_per_update(const plb_ctx* c)
     // FP16 storage with stochastic rounding also loads each node's 4-byte V-order key
     const bool key = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC &&
                      (c->perm || c->key_perm);
-    return 2.0 * Q * (double)c->elem + c->index_bytes + (key ? 4.0 : 0.0);
+    // AA streaming reads the table on odd steps only
+    const double idx = c->cfg.stream_mode == PLB_STREAM_AA ? 0.5 * c->index_bytes : c->index_bytes;
+    return 2.0 * Q * (double)c->elem + idx + (key ? 4.0 : 0.0);
 }
 
 static const int64_t CHUNK_NODES = 1 << 20;
+
 
 extern "C" int plb_read_macros(plb_ctx* c, double* rho, double* ux, double* uy, double* uz)
 {

@@ -166,6 +166,7 @@ extern "C" int plb_batch_create(plb_batch** out, const plb_config* cfg, const pl
     if (n_samples < 1 || !samples) return PLB_ERR_INVALID;
     if (cfg->nodes_per_thread != 1) return PLB_ERR_INVALID;
     if (cfg->index_kind == PLB_INDEX_BITMAP) return PLB_ERR_INVALID;  // samples have their own dims
+    if (cfg->stream_mode != PLB_STREAM_PULL) return PLB_ERR_INVALID;  // AA: single lattices only
     for (int s = 0; s < n_samples; s++) {
         const plb_sample& p = samples[s];
         if (!p.flag || p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || p.nx > 65535 || p.ny > 65535 ||

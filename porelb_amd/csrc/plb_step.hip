@@ -262,15 +262,34 @@ __device__ __forceinline__ void gather(const StepArgs& a, int64_t i, const int32
     }
 }
 
+// Where the post-collision population j of node i is stored.  MODE 0: the
+// other buffer, slot j (pull, two buffers).  One buffer with the AA pattern:
+// MODE 1 (local steps) the node's own slot re[j]; MODE 2 (neighbour steps)
+// slot j of the node downstream, t[re[j]] (x + e_j), or the node's own slot
+// re[j] when that neighbour is solid (half-way bounce-back).  The rest
+// population stays in slot 0.
+template <int LAYOUT, int MODE>
+__device__ __forceinline__ int64_t out_pos(const StepArgs& a, int64_t i, const int32_t t[Q], int j)
+{
+    if constexpr (MODE == 0) return fpos<LAYOUT>(j, i, a.Npad);
+    else if constexpr (MODE == 1) return fpos<LAYOUT>(RE[j], i, a.Npad);
+    else {
+        if (j == 0) return fpos<LAYOUT>(0, i, a.Npad);
+        return t[RE[j]] >= 0 ? fpos<LAYOUT>(j, t[RE[j]], a.Npad) : fpos<LAYOUT>(RE[j], i, a.Npad);
+    }
+}
+
 // FP64 collision in the reference's evaluation order (porelb.c:326-362).
-template <int LAYOUT>
+template <int LAYOUT, int MODE = 0>
 __device__ __forceinline__ void collide_f64(const StepArgs& a, int64_t i, const int32_t t[Q],
                                             double f[Q], unsigned long long fk)
 {
     double* __restrict__ fout = (double*)a.fout;
+    if constexpr (MODE == 0) {  // constant entries: reference bounce order, two buffers only
 #pragma unroll
-    for (int j = 1; j < Q; j++)
-        if (t[j] == TAB_CONST) f[j] = TP[j] * a.rho0 * 1.0;  // feq(rho0, u=0) = initial f1
+        for (int j = 1; j < Q; j++)
+            if (t[j] == TAB_CONST) f[j] = TP[j] * a.rho0 * 1.0;  // feq(rho0, u=0) = initial f1
+    }
 
     double rho = 0.0, rhou = 0.0, rhov = 0.0, rhow = 0.0;
 #pragma unroll
@@ -296,7 +315,7 @@ __device__ __forceinline__ void collide_f64(const StepArgs& a, int64_t i, const 
         const double o = (1.0 - omega) * f[j] +
                          (omega + (1.0 - 0.5 * omega) * 3.0 * (EX[j] - u) * Xforce) *
                              feq_ref(j, rho, u, v, w);
-        __builtin_nontemporal_store(o, fout + fpos<LAYOUT>(j, i, a.Npad));
+        __builtin_nontemporal_store(o, fout + out_pos<LAYOUT, MODE>(a, i, t, j));
     }
 }
 
@@ -343,7 +362,7 @@ __device__ __forceinline__ void st_dev_sr(__half* p, int64_t k, float v, uint32_
     __builtin_nontemporal_store(__builtin_bit_cast(unsigned short, h[0]), (unsigned short*)p + k);
 }
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int MODE = 0>
 __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, const int32_t t[Q],
                                               const S raw[Q], unsigned long long fk, uint32_t key)
 {
@@ -353,10 +372,12 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     float ds[Q];
 #pragma unroll
     for (int j = 0; j < Q; j++) ds[j] = to_float(raw[j]);
-    if (__builtin_amdgcn_readfirstlane(a.has_const)) {  // constant entries: reference bounce order only
+    if constexpr (MODE == 0) {
+        if (__builtin_amdgcn_readfirstlane(a.has_const)) {  // constant entries: reference bounce order only
 #pragma unroll
-        for (int j = 1; j < Q; j++)
-            if (t[j] == TAB_CONST) ds[j] = 0.f;
+            for (int j = 1; j < Q; j++)
+                if (t[j] == TAB_CONST) ds[j] = 0.f;
+        }
     }
 
     float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
@@ -417,10 +438,18 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
             const float feq = fmaf(wj, rho0, dfeq);
             const float frc = EX[j] > 0 ? fps : (EX[j] < 0 ? fms : f0s);
             const float o = fmaf(c1, ds[j], fmaf(omega_s, dfeq, frc * feq));
-            if constexpr (SR)
-                st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, rnd + (uint32_t)j * 0x9e3779b9u);
-            else
-                st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
+            if constexpr (MODE == 0) {
+                if constexpr (SR)
+                    st_dev_sr(onode, fpos<LAYOUT>(j, 0, a.Npad), o, rnd + (uint32_t)j * 0x9e3779b9u);
+                else
+                    st_dev(onode, fpos<LAYOUT>(j, 0, a.Npad), o);
+            } else {
+                const int64_t k = out_pos<LAYOUT, MODE>(a, i, t, j);
+                if constexpr (SR)
+                    st_dev_sr(fout, k, o, rnd + (uint32_t)j * 0x9e3779b9u);
+                else
+                    st_dev(fout, k, o);
+            }
         }
     };
     if constexpr (scaled) {
@@ -442,14 +471,14 @@ __device__ __forceinline__ uint32_t sr_key(const StepArgs& a, int64_t i)
     return (uint32_t)i;
 }
 
-template <typename S, int LAYOUT>
+template <typename S, int LAYOUT, int MODE = 0>
 __device__ __forceinline__ void collide_any(const StepArgs& a, int64_t i, const int32_t t[Q], S raw[Q],
                                             unsigned long long fk, uint32_t key)
 {
     if constexpr (sizeof(S) == 8)
-        collide_f64<LAYOUT>(a, i, t, (double*)raw, fk);
+        collide_f64<LAYOUT, MODE>(a, i, t, (double*)raw, fk);
     else
-        collide_shift<S, LAYOUT>(a, i, t, raw, fk, key);
+        collide_shift<S, LAYOUT, MODE>(a, i, t, raw, fk, key);
 }
 
 template <typename S, int LAYOUT, int IDX, bool CONST_DESC = false>
@@ -472,6 +501,58 @@ __global__ void __launch_bounds__(1024) k_step(const StepArgs a)
     const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
     node_any<S, LAYOUT, IDX>(with_step(a), i, *a.fault);
+}
+
+// Single lattice, one buffer, AA pattern (half-way bounce-back).  The
+// buffer alternates between two forms of the post-collision populations
+// f*_j(y) of the last step: after a neighbour step (and at rest), f*_j(y)
+// lies at out_pos<2>(y, j), which is the pre-collision population of the
+// node downstream -- so the next step reads each node's own 19 slots (MODE
+// 1, no pull table) and leaves f*_j(y) in the node's own slot re[j]
+// (out_pos<1>).  The step after that (MODE 2) reads population j of node x
+// from slot re[j] of the upstream node t[j] (x - e_j; its own slot j at a
+// wall) and stores by out_pos<2>.  A node of a MODE 2 step reads and writes
+// the same 19 locations, so nodes never race; every step stores the values
+// the pull step computes.  c->cur names the form (1: out_pos<1>, 0:
+// out_pos<2>); read-backs map through it.
+template <typename S, int LAYOUT, int IDX, int MODE>
+__device__ __forceinline__ void step_aa(const StepArgs& a0)
+{
+    const StepArgs a = with_step(a0);
+    const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t i = blk * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const S* __restrict__ f = (const S*)a.fin;
+    S raw[Q];
+    if constexpr (MODE == 1) {
+        const uint32_t key = sr_key<S>(a, i);
+#pragma unroll
+        for (int j = 0; j < Q; j++) raw[j] = f[fpos<LAYOUT>(j, i, a.Npad)];
+        const int32_t none[Q] = {};
+        collide_any<S, LAYOUT, 1>(a, i, none, raw, *a.fault, key);
+    } else {
+        Entries<IDX> e;
+        issue_entries<LAYOUT, IDX>(a, i, e);
+        const uint32_t key = sr_key<S>(a, i);
+        decode_entries<IDX>(a, i, e);
+        raw[0] = f[fpos<LAYOUT>(0, i, a.Npad)];
+#pragma unroll
+        for (int j = 1; j < Q; j++) {
+            const int64_t src = e.t[j] >= 0 ? fpos<LAYOUT>(RE[j], e.t[j], a.Npad) : fpos<LAYOUT>(j, i, a.Npad);
+            raw[j] = f[src];
+        }
+        collide_any<S, LAYOUT, 2>(a, i, e.t, raw, *a.fault, key);
+    }
+}
+template <typename S, int LAYOUT, int IDX>
+__global__ void __launch_bounds__(1024) k_step_aa_even(const StepArgs a)
+{
+    step_aa<S, LAYOUT, IDX, 1>(a);
+}
+template <typename S, int LAYOUT, int IDX>
+__global__ void __launch_bounds__(1024) k_step_aa_odd(const StepArgs a)
+{
+    step_aa<S, LAYOUT, IDX, 2>(a);
 }
 
 // Single lattice, two nodes per thread: each wave covers two consecutive
@@ -848,6 +929,77 @@ __global__ void k_copy_state(void* f, int64_t Npad, const uint32_t* inv_perm, in
     }
 }
 
+// The same three copies for a one-buffer AA lattice: population j of node k
+// sits at out_pos<form>(k, j), form 1 or 2 (the step kernels' store map).
+template <int LAYOUT, int IDX>
+__device__ __forceinline__ void aa_positions(const StepArgs& a, int64_t k, int form, int64_t pos[Q]);
+
+template <typename S, int LAYOUT, int IDX>
+__global__ void k_read_pop_aa(const StepArgs a, int form, const uint32_t* inv_perm, int64_t first, int64_t n,
+                              double rho0, float inv, double* out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    int64_t pos[Q];
+    aa_positions<LAYOUT, IDX>(a, k, form, pos);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        double v;
+        if constexpr (sizeof(S) == 8)
+            v = ((const double*)a.fin)[pos[j]];
+        else if constexpr (sizeof(S) == 4)
+            v = TP[j] * rho0 + (double)((const float*)a.fin)[pos[j]];
+        else
+            v = TP[j] * rho0 + (double)(__half2float(((const __half*)a.fin)[pos[j]]) * inv);
+        out[r * Q + j] = v;
+    }
+}
+
+template <typename S, int LAYOUT, int IDX>
+__global__ void k_write_pop_aa(const StepArgs a, int form, const uint32_t* inv_perm, int64_t first, int64_t n,
+                               double rho0, float sc, const double* in)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    int64_t pos[Q];
+    aa_positions<LAYOUT, IDX>(a, k, form, pos);
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const double v = in[r * Q + j];
+        if constexpr (sizeof(S) == 8)
+            ((double*)a.fout)[pos[j]] = v;
+        else if constexpr (sizeof(S) == 4)
+            ((float*)a.fout)[pos[j]] = (float)(v - TP[j] * rho0);
+        else
+            ((__half*)a.fout)[pos[j]] = __float2half_rn((float)(v - TP[j] * rho0) * sc);
+    }
+}
+
+template <typename S, int LAYOUT, int IDX>
+__global__ void k_copy_state_aa(const StepArgs a, int form, const uint32_t* inv_perm, int64_t first, int64_t n,
+                                void* aos_bytes, int to_device)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    int64_t pos[Q];
+    aa_positions<LAYOUT, IDX>(a, k, form, pos);
+    S* dev = (S*)a.fout;
+    S* aos = (S*)aos_bytes;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        if (to_device)
+            dev[pos[j]] = aos[r * Q + j];
+        else
+            aos[r * Q + j] = dev[pos[j]];
+    }
+}
+
 __global__ void k_read_macros(const double* macros, int64_t Npad, const uint32_t* inv_perm,
                               int64_t first, int64_t n, int comp, double* out)
 {
@@ -858,15 +1010,11 @@ __global__ void k_read_macros(const double* macros, int64_t Npad, const uint32_t
     out[r] = macros[comp * Npad + k];
 }
 
+// Pull sources t[1..18] of internal node k, decoded by one lane on its own
+// (the lanes of a wave may hold unrelated nodes).
 template <int LAYOUT, int IDX>
-__global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const uint32_t* perm,
-                             int64_t first, int64_t n, int32_t* out)
+__device__ __forceinline__ void lane_sources(const StepArgs& a, int64_t k, int32_t t[Q])
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int64_t g = first + r;
-    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
-    int32_t t[Q];
     if constexpr (IDX == PLB_INDEX_I32) {
         Entries<IDX> e;
         issue_entries<LAYOUT, IDX>(a, k, e);
@@ -896,6 +1044,33 @@ __global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const u
             }
         }
     }
+}
+
+template <int LAYOUT, int IDX>
+__device__ __forceinline__ void aa_positions(const StepArgs& a, int64_t k, int form, int64_t pos[Q])
+{
+    if (form == 1) {
+#pragma unroll
+        for (int j = 0; j < Q; j++) pos[j] = out_pos<LAYOUT, 1>(a, k, nullptr, j);
+    } else {
+        int32_t t[Q];
+        t[0] = 0;
+        lane_sources<LAYOUT, IDX>(a, k, t);
+#pragma unroll
+        for (int j = 0; j < Q; j++) pos[j] = out_pos<LAYOUT, 2>(a, k, t, j);
+    }
+}
+
+template <int LAYOUT, int IDX>
+__global__ void k_read_table(const StepArgs a, const uint32_t* inv_perm, const uint32_t* perm,
+                             int64_t first, int64_t n, int32_t* out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t g = first + r;
+    const int64_t k = inv_perm ? (int64_t)inv_perm[g] : g;
+    int32_t t[Q];
+    lane_sources<LAYOUT, IDX>(a, k, t);
     for (int j = 1; j < Q; j++) {
         int32_t v = t[j];
         if (v >= 0 && perm) v = (int32_t)perm[v];
@@ -1111,7 +1286,16 @@ int launch_step(plb_ctx* c, int64_t it, bool write_macros, const int64_t* it_bas
     a.it_base = it_base;
     const int bs = c->cfg.block_size;
     const unsigned lds = step_lds_bytes(c, bs);
-    if (c->cfg.nodes_per_thread == 2) {
+    if (c->cfg.stream_mode == PLB_STREAM_AA) {
+        // one buffer: c->cur 1 = local-step store form (a neighbour step
+        // next), 0 = neighbour-step store form (a local step next)
+        if (c->cur == 0)
+            PLB_DISPATCH_ONE_IDX(c->cfg.precision, c->cfg.layout, PLB_INDEX_I32, k_step_aa_even,
+                                 dim3(nblk(c->N, bs)), dim3(bs), lds, c->stream, a);
+        else
+            PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step_aa_odd, dim3(nblk(c->N, bs)),
+                          dim3(bs), lds, c->stream, a);
+    } else if (c->cfg.nodes_per_thread == 2) {
         const int64_t threads = (c->N + 127) / 128 * 64;
         PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_step2,
                       dim3(nblk(threads, bs)), dim3(bs), lds, c->stream, a);
@@ -1244,6 +1428,13 @@ int launch_init_eq(plb_ctx* c)
 
 int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout)
 {
+    if (c->cfg.stream_mode == PLB_STREAM_AA) {
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_read_pop_aa, dim3(nblk(n, 256)),
+                      dim3(256), 0, c->stream, make_args(c, 0, false), c->cur == 1 ? 1 : 2, c->inv_perm, first, n,
+                      c->cfg.rho0, c->inv_scale, dout);
+        PLB_HIP(hipGetLastError());
+        return PLB_OK;
+    }
     PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_read_pop, dim3(nblk(n, 256)), dim3(256), 0,
                  c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, c->cfg.rho0,
                  c->inv_scale, dout);
@@ -1253,6 +1444,13 @@ int launch_read_pop(plb_ctx* c, int64_t first, int64_t n, double* dout)
 
 int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din)
 {
+    if (c->cfg.stream_mode == PLB_STREAM_AA) {
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_write_pop_aa, dim3(nblk(n, 256)),
+                      dim3(256), 0, c->stream, make_args(c, 0, false), c->cur == 1 ? 1 : 2, c->inv_perm, first, n,
+                      c->cfg.rho0, c->scale, din);
+        PLB_HIP(hipGetLastError());
+        return PLB_OK;
+    }
     PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_write_pop, dim3(nblk(n, 256)), dim3(256), 0,
                  c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, c->cfg.rho0, c->scale,
                  din);
@@ -1262,6 +1460,13 @@ int launch_write_pop(plb_ctx* c, int64_t first, int64_t n, const double* din)
 
 int launch_copy_state(plb_ctx* c, int64_t first, int64_t n, void* aos, bool to_device)
 {
+    if (c->cfg.stream_mode == PLB_STREAM_AA) {
+        PLB_DISPATCH3(c->cfg.precision, c->cfg.layout, c->cfg.index_kind, k_copy_state_aa, dim3(nblk(n, 256)),
+                      dim3(256), 0, c->stream, make_args(c, 0, false), c->cur == 1 ? 1 : 2, c->inv_perm, first, n,
+                      aos, to_device ? 1 : 0);
+        PLB_HIP(hipGetLastError());
+        return PLB_OK;
+    }
     PLB_DISPATCH(c->cfg.precision, c->cfg.layout, k_copy_state, dim3(nblk(n, 256)), dim3(256), 0,
                  c->stream, c->f[c->cur], c->Npad, c->inv_perm, first, n, aos, to_device ? 1 : 0);
     PLB_HIP(hipGetLastError());
