@@ -223,6 +223,8 @@ int main(int argc, char *argv[])
         env_int("PLB_WAVES", cfg.precision != PLB_FP16 && v.num_node >= (1 << 22) ? 5 : 0);
     cfg.fp16_rounding = env_choice("PLB_FP16_ROUND", "nearest", "stochastic", NULL, PLB_ROUND_STOCHASTIC);
     cfg.placement_reserve_gib = env_int("PLB_RESERVE_GIB", 0);
+    /* one population buffer (AA pattern): needs PLB_BB=halfway */
+    cfg.stream_mode = env_choice("PLB_STREAM", "pull", "aa", NULL, PLB_STREAM_PULL);
     cfg.device = env_int("PLB_DEVICE", 0);
     const int save_every = env_int("PLB_SAVE", 1000); /* porelb.c:93 */
     const int print = env_int("PLB_PRINT", 1);

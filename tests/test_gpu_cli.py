@@ -196,6 +196,26 @@ def test_porelb_async_saves_identical(tmp_path):
         assert np.array_equal(a, b), it
 
 
+def test_porelb_aa_stream_identical(tmp_path):
+    """PLB_STREAM=aa (one population buffer) writes the part files and the
+    checkpoint of PLB_STREAM=pull, half-way bounce-back, at every save step."""
+    flag = (np.random.default_rng(37).random((16, 12, 14)) < 0.6).astype(np.uint8)
+    nz, ny, nx = flag.shape
+    flag.tofile(tmp_path / "flag.bin")
+    assert run([os.path.join(BIN, "preprocess.x"), "2", str(nx), str(ny), str(nz)], tmp_path).returncode == 0
+    env = {"PLB_TAU": "0.9", "PLB_GX": "1e-5", "PLB_PRINT": "0", "PLB_SAVE": "7", "PLB_BB": "halfway",
+           "PLB_PRECISION": "fp32", "PLB_CHECKPOINT": "ck.bin"}
+    for mode in ("pull", "aa"):
+        wd = tmp_path / mode
+        wd.mkdir()
+        os.link(tmp_path / "V.bin", wd / "V.bin")
+        r = run([os.path.join(BIN, "porelb.x"), "45"], wd, dict(env, PLB_STREAM=mode))
+        assert r.returncode == 0, r.stderr
+    for it in list(range(0, 45, 7)) + [45]:
+        assert np.array_equal(read_parts(tmp_path / "pull", 2, it), read_parts(tmp_path / "aa", 2, it)), it
+    assert (tmp_path / "pull" / "ck.bin").read_bytes() == (tmp_path / "aa" / "ck.bin").read_bytes()
+
+
 @pytest.mark.skipif(not os.path.exists("/dev/full"), reason="needs /dev/full")
 @pytest.mark.parametrize("mode", ["0", "1"])
 def test_porelb_part_file_write_failure(tmp_path, mode):
