@@ -103,3 +103,25 @@ def test_random_shape_fp16_within_rounding(plb, k):
         "max |u|", umax)
     assert np.abs(got[0] - mac[0]).max() <= 4e-6
     assert np.abs(got[1:] - mac[1:]).max() <= 1e-2 * umax + 1e-9
+
+
+AA_CASES = [k for k in range(N_CASES) if draw_case(k)[2] == O.BB_HALFWAY]
+
+
+@pytest.mark.parametrize("k", AA_CASES)
+def test_random_shape_aa_fp64_bitwise(plb, k):
+    """The half-way cases again with one-buffer AA streaming (one node per
+    thread, no graph replays): the oracle's populations and macros bit for
+    bit after an odd number of steps, read back through the AA store map."""
+    flag, np_, mode, kw, tau, gx = draw_case(k)
+    kw = {n: v for n, v in kw.items() if n not in ("nodes_per_thread", "graph_steps")}
+    say(k, "aa", flag.shape[::-1], "np", np_, kw)
+    V, start = O.build_V(flag, np_)
+    tab = O.pull_table(V, start, mode)
+    f0, _ = O.init_equilibrium(len(V))
+    mac = O.pull_run(tab, f0, 17, tau, gx)
+    with plb.Lattice.from_flag(flag, np_, tau=tau, gx=gx, bb_mode=mode, stream_mode=plb.STREAM_AA, **kw) as lat:
+        lat.step(0, 8)
+        lat.step(8, 9)
+        assert np.array_equal(lat.read_populations(), f0)
+        assert np.array_equal(lat.read_macros(), mac)
