@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
     // iff aprefix[s + 1] > aprefix[s]); rebuilt after every decision barrier.
     __shared__ int aprefix[PERSIST_MAX_SAMPLES + 1];
     __shared__ int stop;
-    __shared__ int64_t claim;   // the chunk this block claimed next, in its stretch
+    __shared__ int64_t claim[2];   // the chunks this block claimed, in its stretch (alternating slots)
     const int bs = blockDim.x;
     // The active chunks, in sample order, are split into eight contiguous
     // stretches, one per XCD (block id mod 8).  The blocks of a stretch take
@@ -782,14 +782,16 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
                                __HIP_MEMORY_SCOPE_AGENT);
         const int64_t len = j1 - j0;
         if (threadIdx.x == 0)
-            claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            claim[0] = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        int64_t mine = claim;
+        int cs = 0;   // claim slot being processed
+        int64_t mine = claim[0];
         int s = 0;
         while (mine < len) {
-            __syncthreads();   // every thread has read the claim before it is replaced
+            // The other slot was last read before the previous barrier.
             if (threadIdx.x == 0)
-                claim = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                claim[cs ^ 1] = (int64_t)__hip_atomic_fetch_add(claim_cnt, 1ull, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
             const int64_t j = j0 + mine;
             while (aprefix[s + 1] <= j) s++;   // uniform per block; a block's claims only grow
             const int64_t chunk = ld_const(p.sample_off + s) / bs + (j - aprefix[s]);
@@ -805,7 +807,8 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
                 if (threadIdx.x == 0) p.partial[chunk] = t_sum;
             }
             __syncthreads();   // the next claim has landed in LDS
-            mine = claim;
+            cs ^= 1;
+            mine = claim[cs];
         }
         if (!grid_barrier(p, gen)) return;
         // Density guard: stop after the faulting step.  Only a fault of this
