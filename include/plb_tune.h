@@ -32,8 +32,9 @@ typedef struct {
  * for half-way bounce-back) x layout (SOA, AOSOA) x node order (reference;
  * 8^3 bricks; pencils of 64 and 128 full rows) at 256 threads, each on a
  * context of its own.  Stage 2: block size (128, 256, 512) x nodes per thread
- * (1, 2) x waves per SIMD (no cap, 6, 5, 4) for the stage-1 winner, all timed
- * on one context, since they change only the launch.  All candidates give
+ * (1, 2; 1 only for AA streaming) x waves per SIMD (no cap, 6, 5, 4) for the
+ * stage-1 winner, all timed on one context, since they change only the
+ * launch.  The stream mode of base is kept.  All candidates give
  * bit-identical results.  When cache_path
  * is not NULL, a result stored there for the same mask (64-bit hash of the
  * voxels and dimensions), precision and bounce-back mode is returned without

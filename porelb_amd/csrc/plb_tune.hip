@@ -117,7 +117,9 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
     if (n_results) *n_results = 0;
     *best = *base;
     const int64_t nvox = (int64_t)nx * ny * nz;
-    const uint64_t key = mask_hash(flag, nvox, nx, ny, nz);
+    // AA streaming is tuned and cached apart from the pull mode
+    const uint64_t key = mask_hash(flag, nvox, nx, ny, nz) ^
+                         (base->stream_mode == PLB_STREAM_AA ? 0x9e3779b97f4a7c15ull : 0ull);
     if (cache_path && cache_lookup(cache_path, key, base->precision, base->bb_mode, best))
         return PLB_OK;
 
@@ -187,6 +189,7 @@ extern "C" int plb_autotune(const plb_config* base, const uint8_t* flag, int nx,
     for (int bs : {128, 256, 512})
         for (int npt : {1, 2})
             for (int waves : {0, 6, 5, 4}) {
+                if (npt == 2 && stage1.stream_mode == PLB_STREAM_AA) continue;  // AA: one node per thread
                 plb_config cfg = stage1;
                 cfg.block_size = bs;
                 cfg.nodes_per_thread = npt;

@@ -39,3 +39,25 @@ def test_autotune_cache_and_result(plb, tmp_path):
     with plb.Lattice.from_flag(flag, 1, cfg=best) as lat:
         lat.step(0, 30)
         assert np.array_equal(lat.read_populations(), f0)
+
+
+def test_autotune_aa_keeps_stream_mode(plb, tmp_path):
+    """Tuning a one-buffer AA configuration returns an AA configuration with
+    one node per thread (the only form AA runs), cached apart from the pull
+    mode, and its result equals the oracle's half-way run."""
+    flag = plb.mask_spheres((40, 36, 48), 3.0, 6.0, 0.3, 9)
+    cache = str(tmp_path / "tune.txt")
+    pull = plb.default_config(tau=0.8, gx=1e-5, precision=plb.FP64, bb_mode=plb.BB_HALFWAY)
+    aa = plb.default_config(tau=0.8, gx=1e-5, precision=plb.FP64, bb_mode=plb.BB_HALFWAY,
+                            stream_mode=plb.STREAM_AA)
+    plb.autotune(flag, 1, 4, cache, cfg=pull)
+    best, res = plb.autotune(flag, 1, 4, cache, cfg=aa)
+    assert res and all(r["npt"] == 1 for r in res)
+    assert best.stream_mode == plb.STREAM_AA and best.nodes_per_thread == 1
+    V, start = O.build_V(flag, 1)
+    tab = O.pull_table(V, start, O.BB_HALFWAY)
+    f0, _ = O.init_equilibrium(len(V))
+    O.pull_run(tab, f0, 21, 0.8, 1e-5)
+    with plb.Lattice.from_flag(flag, 1, cfg=best) as lat:
+        lat.step(0, 21)
+        assert np.array_equal(lat.read_populations(), f0)
