@@ -725,29 +725,31 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(StepArgs a, const Per
     const int64_t ngrp = G < 8 ? G : 8;   // groups that have blocks
     unsigned long long gen = 0;
     // Residency check before any work: every block counts itself in and
-    // waits, briefly, for the whole grid.  A grid larger than what the CUs
-    // really hold never completes the count; the blocks then leave with
-    // abort = 2 and the host launches a smaller grid.
+    // waits, briefly, for the whole grid.  One decision word settles the
+    // outcome for all blocks: the first block to see the full count sets it
+    // to "go", the first to wait past start_ticks sets it to "leave" (a grid
+    // larger than what the CUs really hold never completes the count), and
+    // every block then follows it.  On "leave" the blocks exit before any
+    // work with abort = 2, and the host launches a smaller grid.
     if (threadIdx.x == 0) {
-        int good = 0;
         unsigned long long* started = p.bar + 33 * BAR_STRIDE;
-        if (!__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            __hip_atomic_fetch_add(started, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint64_t t0 = wall_clock64();
-            for (;;) {
-                if (__hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G) {
-                    good = 1;
-                    break;
-                }
-                if (__hip_atomic_load(p.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > p.start_ticks) {
-                    __hip_atomic_store(p.abort, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
+        unsigned long long* decision = started + 1;   // 0 open, 1 go, 2 leave
+        __hip_atomic_fetch_add(started, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = wall_clock64();
+        unsigned long long d;
+        while ((d = __hip_atomic_load(decision, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+            unsigned long long open = 0;
+            if (__hip_atomic_load(started, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= G)
+                __hip_atomic_compare_exchange_strong(decision, &open, 1ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+            else if (wall_clock64() - t0 > p.start_ticks)
+                __hip_atomic_compare_exchange_strong(decision, &open, 2ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+            else
                 __builtin_amdgcn_s_sleep(8);
-            }
         }
-        stop = !good;
+        if (d == 2) __hip_atomic_store(p.abort, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stop = d != 1;
     }
     __syncthreads();
     if (stop) return;
