@@ -2,8 +2,10 @@
 step stores exactly what the two-buffer pull step stores, in every
 precision, index encoding, layout and node order; populations and stored
 elements read back in the pull layout after odd and even step counts; a
-state written into an AA lattice resumes bit for bit; a density-guard trip
-freezes the same state; FP64 equals the oracle's half-way pull run."""
+state written into an AA lattice resumes bit for bit, also when it crosses
+between the stream modes; the convergence driver stops at the same step; a
+density-guard trip freezes the same state; FP64 equals the oracle's half-way
+pull run."""
 import os
 
 import numpy as np
@@ -73,6 +75,52 @@ def test_aa_state_resume(plb, split):
         a.write_state(snap)
         a.step(split, 20 - split)
         assert np.array_equal(a.read_state(), ref.read_state())
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+@pytest.mark.parametrize("split", [7, 8])
+def test_state_crosses_stream_modes(plb, prec, split):
+    """Stored elements are the same in both stream modes: a state read from a
+    pull lattice continues in an AA lattice, and one read from an AA lattice
+    continues in a pull lattice, each like a straight pull run."""
+    flag = mask(plb)
+    kw = dict(tau=0.75, gx=3e-5, precision=prec, bb_mode=plb.BB_HALFWAY, index_kind=1, layout=1, ordering=2,
+              brick=(30, 4, 1), fp16_rounding=plb.ROUND_STOCHASTIC)
+    say(f"prec {prec} split {split}")
+    with plb.Lattice.from_flag(flag, **kw) as ref, \
+            plb.Lattice.from_flag(flag, **kw) as pull, \
+            plb.Lattice.from_flag(flag, stream_mode=plb.STREAM_AA, **kw) as aa:
+        ref.step(0, split)
+        snap = ref.read_state()
+        ref.step(split, 20 - split)
+        aa.write_state(snap)
+        aa.step(split, 20 - split)
+        assert np.array_equal(aa.read_state(), ref.read_state())
+        aa.init_equilibrium()
+        aa.step(0, split)
+        pull.write_state(aa.read_state())
+        pull.step(split, 20 - split)
+        assert np.array_equal(pull.read_state(), ref.read_state())
+
+
+@pytest.mark.parametrize("check", [25, 24])
+def test_aa_run_to_convergence_equals_pull(plb, check):
+    """The convergence driver stops an AA lattice at the same step, with the
+    same sums and stored elements, as a pull lattice (check intervals of 25
+    and 24 steps)."""
+    flag = mask(plb)
+    kw = dict(tau=0.9, gx=1e-5, precision=1, bb_mode=plb.BB_HALFWAY, index_kind=1, layout=1, ordering=2,
+              brick=(30, 4, 1))
+    with plb.Lattice.from_flag(flag, **kw) as pull, \
+            plb.Lattice.from_flag(flag, stream_mode=plb.STREAM_AA, **kw) as aa:
+        n_p, r_p = pull.run_to_convergence(0, 20000, check, 1e-4)
+        n_a, r_a = aa.run_to_convergence(0, 20000, check, 1e-4)
+        say(f"check {check}: stopped after {n_p} steps")
+        assert 0 < n_p < 20000
+        assert n_a == n_p
+        assert r_a["sum_ux"] == r_p["sum_ux"]
+        assert np.array_equal(aa.read_state(), pull.read_state())
+        assert np.array_equal(aa.read_macros(), pull.read_macros())
 
 
 def test_aa_fp64_equals_oracle(plb):
