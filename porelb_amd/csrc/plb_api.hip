@@ -165,6 +165,11 @@ int plb::ctx_alloc_state(plb_ctx* c)
     PLB_HIP(hipMalloc(&c->red, sizeof(double) * (2 * c->red_blocks + 2)));
     PLB_HIP(hipMalloc(&c->it_dev, sizeof(int64_t)));
     PLB_HIP(hipStreamSynchronize(c->stream));
+    if (c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC) {
+        c->key_bytes = 4.0;
+        const int rc = build_key_runs(c);
+        if (rc) return rc;
+    }
     return plb_init_equilibrium(c);
 }
 
@@ -176,7 +181,7 @@ extern "C" void plb_destroy(plb_ctx* c)
         (void)hipStreamSynchronize(c->stream);
     }
     void* ptrs[] = {c->f[0], c->f[1] != c->f[0] ? c->f[1] : nullptr, c->tab, c->tdesc, c->toff, c->toff16, c->twide, c->bm, c->vox, c->perm,
-                    c->inv_perm, c->key_perm, c->node_xyz, c->macros, c->fault, c->red, c->reserve};
+                    c->inv_perm, c->key_perm, c->kseg, c->node_xyz, c->macros, c->fault, c->red, c->reserve};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -386,12 +391,13 @@ extern "C" void plb_host_free(void* p)
 extern "C" double plb_bytes_per_update(const plb_ctx* c)
 {
     if (!c) return 0.0;
-    // FP16 storage with stochastic rounding also loads each node's 4-byte V-order key
+    // FP16 storage with stochastic rounding also reads each node's V-order key:
+    // 16 bytes per tile of runs, plus 4 per node in tiles that load their keys
     const bool key = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC &&
                      (c->perm || c->key_perm);
     // AA streaming reads the table on odd steps only
     const double idx = c->cfg.stream_mode == PLB_STREAM_AA ? 0.5 * c->index_bytes : c->index_bytes;
-    return 2.0 * Q * (double)c->elem + idx + (key ? 4.0 : 0.0);
+    return 2.0 * Q * (double)c->elem + idx + (key ? c->key_bytes : 0.0);
 }
 
 static const int64_t CHUNK_NODES = 1 << 20;

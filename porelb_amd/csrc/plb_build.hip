@@ -595,6 +595,65 @@ int compress_tile8(plb_ctx* c)
     return PLB_OK;
 }
 
+// ---------------------------------------------------------------- FP16 rounding keys
+// The stochastic-rounding key of a node is its V-order id.  In pencil order
+// the rows of one pencil in one z-plane follow each other as in the V order,
+// so a 64-node tile's keys are one or two runs of consecutive values (a run
+// ends where the pencil moves on to the next plane).  A tile with at most
+// three runs (the builder checks every lane) is stored as the runs' bases
+// (key - lane at the run's first lane) and start lanes, 16 bytes that the
+// step reads with one scalar load instead of 4 bytes per node; other tiles
+// are marked (bit 31 of the last word) and keep loading their keys.  Lanes at
+// or past N are not stepped and are skipped.
+__global__ void k_key_runs(const uint32_t* key, int64_t N, int64_t T, uint4* out, unsigned long long* n_marked)
+{
+    const int64_t tile = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= T) return;
+    const int64_t i0 = tile * 64;
+    uint32_t base[3] = {0u, 0u, 0u}, start[3] = {0u, 64u, 64u};
+    int runs = 0;
+    bool fits = true;
+    uint32_t prev = 0u;
+    for (int l = 0; l < 64 && i0 + l < N; l++) {
+        const uint32_t k = key[i0 + l];
+        if (l == 0 || k != prev + 1u) {
+            if (runs == 3) {
+                fits = false;
+                break;
+            }
+            start[runs] = (uint32_t)l;
+            base[runs] = k - (uint32_t)l;
+            runs++;
+        }
+        prev = k;
+    }
+    if (fits) {
+        out[tile] = make_uint4(base[0], base[1], base[2], start[1] | (start[2] << 8));
+    } else {
+        out[tile] = make_uint4(0u, 0u, 0u, 0x80000000u);
+        atomicAdd(n_marked, 1ull);
+    }
+}
+
+int build_key_runs(plb_ctx* c)
+{
+    const uint32_t* key = c->key_perm ? c->key_perm : c->perm;
+    if (!key || c->N == 0) return PLB_OK;
+    const int64_t T = (c->Npad + 63) / 64;
+    DevBuf<unsigned long long> marked;
+    PLB_HIP(marked.alloc(1));
+    PLB_HIP(hipMemsetAsync(marked.p, 0, sizeof(unsigned long long), c->stream));
+    PLB_HIP(hipMalloc(&c->kseg, sizeof(uint4) * T));
+    hipLaunchKernelGGL(k_key_runs, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, c->stream, key, c->N, T,
+                       c->kseg, marked.p);
+    PLB_HIP(hipGetLastError());
+    unsigned long long nm = 0;
+    PLB_HIP(hipMemcpyAsync(&nm, marked.p, sizeof nm, hipMemcpyDeviceToHost, c->stream));
+    PLB_HIP(hipStreamSynchronize(c->stream));
+    c->key_bytes = (16.0 * (double)T + 256.0 * (double)nm) / (double)c->N;
+    return PLB_OK;
+}
+
 // ---------------------------------------------------------------- BITMAP index
 // Node order as a voxel position: pencils of py rows, z fastest across
 // pencils, V = x + nx*(ly + py*(z + nz*yb)) with y = yb*py + ly.  With

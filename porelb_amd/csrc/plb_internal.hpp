@@ -128,6 +128,11 @@ struct plb_ctx {
     double index_bytes = 72.0;  // pull-table bytes per node
     uint32_t* perm = nullptr;   // internal -> V order (null: identity)
     uint32_t* key_perm = nullptr;  // batches: internal -> V order within the node's sample
+    // FP16 stochastic rounding: per 64-node tile, the rounding keys as up to
+    // three runs of consecutive keys {base0, base1, base2, start1 | start2 << 8},
+    // key = base_s + lane in run s; bit 31 of the last word: load key_perm/perm.
+    uint4* kseg = nullptr;       // null: keys loaded per node
+    double key_bytes = 0.0;     // rounding-key bytes per node update (FP16 stochastic)
     uint32_t* inv_perm = nullptr;
     uint64_t* node_xyz = nullptr;  // internal order
     double* macros = nullptr;   // [4][Npad]
@@ -246,6 +251,7 @@ int launch_sample_sums(plb_ctx* c, const int32_t* d_active, int n_active, const 
 int check_cfg(const plb_config* cfg);
 int fp16_scale_log2_for(double gx);
 int compress_tile8(plb_ctx* c);
+int build_key_runs(plb_ctx* c);
 int build_bitmap(plb_ctx* c);
 int compute_order(const plb_config& cfg, int nx, int ny, int nz, int64_t N, const uint64_t* xyz_ref,
                   hipStream_t s, uint32_t** inv_perm, uint32_t** perm, uint64_t* xyz_int);

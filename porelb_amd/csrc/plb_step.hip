@@ -57,6 +57,8 @@ struct StepArgs {
     // (within its sample for batches), so every node order and a batch give
     // the same results as a single lattice in the V order.  Null: identity.
     const uint32_t* key_perm;
+    // The same keys as runs per 64-node tile (null: load key_perm per node).
+    const uint4* kseg;
 };
 
 __device__ __forceinline__ StepArgs with_step(const StepArgs& a)
@@ -462,12 +464,25 @@ __device__ __forceinline__ void collide_shift(const StepArgs& a, int64_t i, cons
     }
 }
 
-// Key of node i's stochastic-rounding stream (FP16 storage only).
+// Key of node i's stochastic-rounding stream (FP16 storage only): from the
+// tile's runs of consecutive keys (one scalar load per wave), or per node in
+// the tiles marked as not fitting three runs.  Called after the gathers are
+// issued, so the scalar load's latency hides behind theirs.
 template <typename S>
 __device__ __forceinline__ uint32_t sr_key(const StepArgs& a, int64_t i)
 {
     if constexpr (sizeof(S) == 2)
-        if (a.stochastic && a.key_perm) return a.key_perm[i];
+        if (a.stochastic && a.key_perm) {
+            if (a.kseg) {
+                const int64_t tile = __builtin_amdgcn_readfirstlane((int)(i >> 6));
+                const uint32_t* d = (const uint32_t*)(a.kseg + tile);
+                const uint32_t b0 = ld_const(d), b1 = ld_const(d + 1), b2 = ld_const(d + 2), st = ld_const(d + 3);
+                const uint32_t lane = (uint32_t)i & 63u;
+                const uint32_t base = lane >= ((st >> 8) & 0xffu) ? b2 : (lane >= (st & 0xffu) ? b1 : b0);
+                if (!(st >> 31)) return base + lane;
+            }
+            return a.key_perm[i];
+        }
     return (uint32_t)i;
 }
 
@@ -486,10 +501,10 @@ __device__ __forceinline__ void node_any(const StepArgs& a, int64_t i, unsigned 
 {
     Entries<IDX> e;
     issue_entries<LAYOUT, IDX, CONST_DESC>(a, i, e);
-    const uint32_t key = sr_key<S>(a, i);
     decode_entries<IDX>(a, i, e);
     S raw[Q];
     gather<S, LAYOUT>(a, i, e.t, raw);
+    const uint32_t key = sr_key<S>(a, i);
     collide_any<S, LAYOUT>(a, i, e.t, raw, fk, key);
 }
 
@@ -525,15 +540,14 @@ __device__ __forceinline__ void step_aa(const StepArgs& a0)
     const S* __restrict__ f = (const S*)a.fin;
     S raw[Q];
     if constexpr (MODE == 1) {
-        const uint32_t key = sr_key<S>(a, i);
 #pragma unroll
         for (int j = 0; j < Q; j++) raw[j] = f[fpos<LAYOUT>(j, i, a.Npad)];
+        const uint32_t key = sr_key<S>(a, i);
         const int32_t none[Q] = {};
         collide_any<S, LAYOUT, 1>(a, i, none, raw, *a.fault, key);
     } else {
         Entries<IDX> e;
         issue_entries<LAYOUT, IDX>(a, i, e);
-        const uint32_t key = sr_key<S>(a, i);
         decode_entries<IDX>(a, i, e);
         raw[0] = f[fpos<LAYOUT>(0, i, a.Npad)];
 #pragma unroll
@@ -541,6 +555,7 @@ __device__ __forceinline__ void step_aa(const StepArgs& a0)
             const int64_t src = e.t[j] >= 0 ? fpos<LAYOUT>(RE[j], e.t[j], a.Npad) : fpos<LAYOUT>(j, i, a.Npad);
             raw[j] = f[src];
         }
+        const uint32_t key = sr_key<S>(a, i);
         collide_any<S, LAYOUT, 2>(a, i, e.t, raw, *a.fault, key);
     }
 }
@@ -574,10 +589,10 @@ __global__ void __launch_bounds__(512) k_step2(const StepArgs a0)
     if (two) issue_entries<LAYOUT, IDX>(a, i1, e1);
     decode_entries<IDX>(a, i0, e0);
     if (two) decode_entries<IDX>(a, i1, e1);
-    const uint32_t k0 = sr_key<S>(a, i0), k1 = two ? sr_key<S>(a, i1) : 0u;
     S r0[Q], r1[Q];
     gather<S, LAYOUT>(a, i0, e0.t, r0);
     if (two) gather<S, LAYOUT>(a, i1, e1.t, r1);
+    const uint32_t k0 = sr_key<S>(a, i0), k1 = two ? sr_key<S>(a, i1) : 0u;
     collide_any<S, LAYOUT>(a, i0, e0.t, r0, fk, k0);
     if (two) collide_any<S, LAYOUT>(a, i1, e1.t, r1, fk, k1);
 }
@@ -1264,6 +1279,7 @@ static StepArgs make_args(plb_ctx* c, int64_t it, bool write_macros)
     a.it_base = nullptr;
     a.stochastic = c->cfg.precision == PLB_FP16 && c->cfg.fp16_rounding == PLB_ROUND_STOCHASTIC;
     a.key_perm = c->key_perm ? c->key_perm : c->perm;
+    a.kseg = c->kseg;
     a.has_const = c->cfg.bb_mode == PLB_BB_REFCOMPAT;
     return a;
 }
